@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark: uint32 inclusive scan (chained scan with decoupled lookback) at
+n = 2^28 full-range values on one MI355X, as GB/s at 8 algorithmic bytes per
+element (4 read + 4 write) and as a fraction of the 8 TB/s HBM peak.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu-baseline]
+
+One step = one scan of the whole 2^28-element input (1 GiB in, 1 GiB out,
+separate buffers, input resident in HBM). Prints ONE JSON line on stdout;
+progress goes to stderr. The CPU baseline is the serial host scan (the
+reference's validator loop restated in C) timed on this machine's host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_ELEM = 8
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def gbps(n: int, ms: float, bytes_per_elem: float = BYTES_PER_ELEM) -> float:
+    return n * bytes_per_elem / (ms * 1e-3) / 1e9
+
+
+def load_traffic(name: str):
+    p = ROOT / "profiles" / f"{name}_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--extras", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--cpu-baseline", action=argparse.BooleanOptionalAction, default=True)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this library is single-GPU (the multi-GPU "
+              "sharding of the scan is a design note, not built); run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import gpuprefixsums_amd as gps
+    import oracle
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    n = 1 << args.log2n
+    K, W = args.steps, args.warmup
+    seed = 2
+
+    log(f"generating n=2^{args.log2n} full-range uint32 input (seed {seed}) on the host")
+    x = oracle.gen_u32(seed, n, 32)
+    d_in = torch.from_numpy(x.view(np.int32)).to("cuda")
+    d_out = torch.empty(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    h = gps.ScanHandle(gps.ALGO_CSDL, n)
+    for _ in range(W):
+        h.inclusive(d_in, d_out, n)
+    torch.cuda.synchronize()
+
+    # Timed region: K scans, each bracketed by hipEvents on the launch stream
+    # (the null stream, torch's current stream); synchronize on both sides.
+    t0 = time.perf_counter()
+    ms = h.time(d_in, d_out, n, True, iters=K)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    errors = gps.validate_scan_u32(d_in, d_out, n, True)
+    dev_err = h.device_error()
+    if errors or dev_err:
+        log(f"FAILED: {errors} validation errors, device error bits {dev_err}")
+        return 1
+    mean_ms = statistics.fmean(ms)
+    med_ms = statistics.median(ms)
+    value = K * n * BYTES_PER_ELEM / wall / 1e9
+    log(f"csdl inclusive: {value:.1f} GB/s whole-job; kernel mean {mean_ms * 1e3:.1f} us, "
+        f"median {med_ms * 1e3:.1f} us -> {gbps(n, mean_ms):.1f} GB/s")
+
+    traffic = load_traffic(f"csdl_u32_n2p{args.log2n}")
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(gbps(n, mean_ms), 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(gbps(n, mean_ms) / HBM_PEAK_GBPS, 4),
+        "traffic": traffic.get("bytes_per_launch") if traffic else None,
+        "kernel": f"csdl_u32<{h.block_threads},{h.vec_per_thread},incl>",
+        "algorithmic_bytes": n * BYTES_PER_ELEM,
+        "kernel_ms_mean": round(mean_ms, 5),
+        "kernel_ms_median": round(med_ms, 5),
+        "kernel_ms_p5_p95": [round(float(np.percentile(ms, 5)), 5), round(float(np.percentile(ms, 95)), 5)],
+    }
+
+    extras = {}
+    if args.extras:
+        def timed(handle, inclusive=True, iters=20):
+            for _ in range(3):
+                handle.scan(d_in, d_out, n, inclusive)
+            t = handle.time(d_in, d_out, n, inclusive, iters=iters)
+            e = gps.validate_scan_u32(d_in, d_out, n, inclusive)
+            if e or handle.device_error():
+                raise RuntimeError("validation failed in extras")
+            return statistics.median(t)
+
+        m = timed(h, inclusive=False)
+        extras["csdl_exclusive_gbps"] = round(gbps(n, m), 1)
+        with gps.ScanHandle(gps.ALGO_RTS, n) as r:
+            m = timed(r)
+            extras["rts_inclusive_gbps"] = round(gbps(n, m), 1)
+        for period in (0, 7, 8):
+            with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period) as f:
+                m = timed(f)
+                c = f.counters()
+                extras[f"csdldf_stall{period}_gbps"] = round(gbps(n, m), 1)
+                extras[f"csdldf_stall{period}_counters_per_launch"] = {
+                    k: round(v / 23, 1) for k, v in c.items()}
+        # copy ceiling: read n + write n with the same access pattern
+        st = torch.cuda.current_stream()
+        for _ in range(3):
+            gps.copy_u32(d_out, d_in, n)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(20):
+            gps.copy_u32(d_out, d_in, n)
+        e1.record(st)
+        torch.cuda.synchronize()
+        copy_ms = e0.elapsed_time(e1) / 20
+        extras["copy_ceiling_gbps"] = round(gbps(n, copy_ms), 1)
+        extras["csdl_frac_of_copy_ceiling"] = round(gbps(n, mean_ms) / gbps(n, copy_ms), 4)
+        # n = 2^20 latency (values [0,255], seed 1)
+        n1 = 1 << 20
+        x1 = torch.from_numpy(oracle.gen_u32(1, n1, 8).view(np.int32)).to("cuda")
+        o1 = torch.empty_like(x1)
+        with gps.ScanHandle(gps.ALGO_CSDL, n1) as h1:
+            for _ in range(5):
+                h1.inclusive(x1, o1, n1)
+            t = h1.time(x1, o1, n1, True, iters=50)
+            extras["n2p20_csdl_inclusive_us"] = round(statistics.median(t) * 1e3, 2)
+        log(f"extras: {json.dumps(extras)}")
+
+    cpu = None
+    if args.cpu_baseline:
+        log("timing the serial host scan (oracle port of the reference's validator loop)")
+        out = np.empty_like(x)
+        runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            oracle.lib().orc_scan_u32(x, out, n, 1)
+            runs.append(time.perf_counter() - t)
+        s = statistics.median(runs)
+        cpu = {"value": round(n * BYTES_PER_ELEM / s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"full workload n=2^{args.log2n} inclusive, median of 3 runs "
+                         f"({s * 1e3:.0f} ms each) on {cpu_model()}"}
+        threads = min(16, os.cpu_count() or 1)
+        runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            oracle.lib().orc_scan_u32_mt(x, out, n, 1, threads)
+            runs.append(time.perf_counter() - t)
+        s = statistics.median(runs)
+        extras["cpu_all_cores"] = {"value": round(n * BYTES_PER_ELEM / s / 1e9, 3), "unit": "GB/s",
+                                   "cores": threads, "kind": "port"}
+        log(f"cpu baseline: {json.dumps(cpu)}")
+
+    line = {
+        "metric": "scan GB/s & %HBM peak, u32 n=2^28; segmented batch vs plain; tuner regret",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(wall / K * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"uint32 inclusive scan, CSDL, n=2^{args.log2n}, values uniform in "
+                               f"[0, 2^32-1] (splitmix64 seed {seed}), separate 1 GiB in/out",
+                   "n": n, "algo": "csdl", "tile": h.tile, "bytes_per_elem": BYTES_PER_ELEM},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    h.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,192 @@
+"""gpuprefixsums_amd: MI355X (gfx950) device-wide prefix sums.
+
+Python host mirror of the C ABI in include/gps.h (libgps.so, hand-written HIP
+kernels). Device buffers are passed as raw device addresses or as any object
+with .data_ptr() (e.g. torch tensors on a ROCm device); PyTorch is only used
+by callers for memory and streams. There is no CPU fallback: if libgps.so is
+not built, importing the binding raises.
+
+Reference interface mirrored (b0nes164/GPUPrefixSums):
+  ChainedScanDecoupledLookbackDispatcher  GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:16-163
+  ReduceThenScanDispatcher                GPUPrefixSumsCUDA/ReduceThenScanDispatcher.cuh:16-152
+  EmulatedDeadlockingDispatcher           GPUPrefixSumsCUDA/EmulatedDeadlockingDispatcher.cuh:19-163
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+from . import _lib
+from ._lib import ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, GpsError, GpsTuning, check, load
+
+__all__ = [
+    "ALGO_CSDL", "ALGO_RTS", "ALGO_CSDLDF", "GpsError", "ScanHandle",
+    "ChainedScanDecoupledLookbackDispatcher", "ReduceThenScanDispatcher",
+    "EmulatedDeadlockingDispatcher", "fill_u32", "fill_random_u32", "validate_scan_u32",
+    "validate_ones_u32", "load",
+]
+
+
+def _ptr(x) -> int:
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    raise TypeError(f"expected a device pointer or tensor, got {type(x)!r}")
+
+
+def _stream(s) -> Optional[int]:
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    if hasattr(s, "cuda_stream"):
+        return int(s.cuda_stream)
+    raise TypeError(f"expected a stream handle, got {type(s)!r}")
+
+
+def _numel(x) -> int:
+    if hasattr(x, "numel"):
+        return int(x.numel())
+    raise TypeError("n must be given for raw pointers")
+
+
+class ScanHandle:
+    """One C-ABI handle: owns the status words / ticket / tile-sum scratch for
+    scans of up to max_n elements. Not re-entrant: use one per stream."""
+
+    def __init__(self, algo: int = ALGO_CSDL, max_n: int = 1 << 28, *, block_threads: int = 0,
+                 vec_per_thread: int = 0, spin_limit: int = 0, stall_period: int = 0):
+        L = load()
+        t = GpsTuning(block_threads, vec_per_thread, spin_limit, stall_period)
+        h = ctypes.c_void_p()
+        check(L.gps_create(algo, max_n, ctypes.byref(t), ctypes.byref(h)))
+        self._h = h
+        self.algo = algo
+        self.max_n = max_n
+        out = GpsTuning()
+        tile = ctypes.c_uint32()
+        check(L.gps_get_tuning(h, ctypes.byref(out), ctypes.byref(tile)))
+        self.block_threads = out.block_threads
+        self.vec_per_thread = out.vec_per_thread
+        self.spin_limit = out.spin_limit
+        self.stall_period = out.stall_period
+        self.tile = tile.value
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError("handle closed")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            check(load().gps_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def inclusive(self, d_in, d_out, n: Optional[int] = None, stream=None) -> None:
+        n = _numel(d_in) if n is None else n
+        check(load().gps_inclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
+
+    def exclusive(self, d_in, d_out, n: Optional[int] = None, stream=None) -> None:
+        n = _numel(d_in) if n is None else n
+        check(load().gps_exclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
+
+    def scan(self, d_in, d_out, n: Optional[int] = None, inclusive: bool = True, stream=None) -> None:
+        (self.inclusive if inclusive else self.exclusive)(d_in, d_out, n, stream)
+
+    def time(self, d_in, d_out, n: int, inclusive: bool = True, iters: int = 10, stream=None) -> list:
+        ms = (ctypes.c_float * iters)()
+        check(load().gps_time_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive),
+                                       _stream(stream), iters, ms))
+        return list(ms)
+
+    def counters(self) -> dict:
+        c = (ctypes.c_uint64 * 4)()
+        check(load().gps_get_counters(self.handle, c))
+        return {"polls": c[0], "fallbacks": c[1], "insertions": c[2], "stall_timeouts": c[3]}
+
+    def reset_counters(self) -> None:
+        check(load().gps_reset_counters(self.handle))
+
+    def device_error(self) -> int:
+        bits = ctypes.c_uint32()
+        check(load().gps_device_error(self.handle, ctypes.byref(bits)))
+        return bits.value
+
+    def reset(self, stream=None) -> None:
+        check(load().gps_reset(self.handle, _stream(stream)))
+
+
+# ---- utility kernels -----------------------------------------------------
+
+def fill_u32(d_ptr, n: int, value: int, stream=None) -> None:
+    check(load().gps_fill_u32(_ptr(d_ptr), n, value & 0xFFFFFFFF, _stream(stream)))
+
+
+def copy_u32(d_dst, d_src, n: int, stream=None) -> None:
+    check(load().gps_copy_u32(_ptr(d_dst), _ptr(d_src), n, _stream(stream)))
+
+
+def fill_random_u32(d_ptr, n: int, seed: int, bits: int = 32, first: int = 0, stream=None) -> None:
+    check(load().gps_fill_random_u32(_ptr(d_ptr), n, seed, first, bits, _stream(stream)))
+
+
+def validate_scan_u32(d_in, d_scan, n: int, inclusive: bool = True, stream=None) -> int:
+    e = ctypes.c_uint64()
+    check(load().gps_validate_scan_u32(_ptr(d_in), _ptr(d_scan), n, int(inclusive), _stream(stream),
+                                       ctypes.byref(e)))
+    return e.value
+
+
+def validate_ones_u32(d_scan, n: int, inclusive: bool = True, stream=None) -> int:
+    e = ctypes.c_uint64()
+    check(load().gps_validate_ones_u32(_ptr(d_scan), n, int(inclusive), _stream(stream), ctypes.byref(e)))
+    return e.value
+
+
+# ---- reference dispatcher classes (self-tests run in the C++ classes) -----
+
+class _Dispatcher:
+    _algo = ALGO_CSDL
+
+    def __init__(self, max_size: int):
+        self.max_size = max_size
+
+    def _test_all(self, inclusive: bool) -> tuple:
+        passed, total = ctypes.c_uint32(), ctypes.c_uint32()
+        check(load().gps_dispatcher_test_all(self._algo, self.max_size, int(inclusive), ctypes.byref(passed),
+                                             ctypes.byref(total)))
+        return passed.value, total.value
+
+    def test_all_inclusive(self) -> tuple:
+        return self._test_all(True)
+
+    def test_all_exclusive(self) -> tuple:
+        return self._test_all(False)
+
+
+class ChainedScanDecoupledLookbackDispatcher(_Dispatcher):
+    _algo = ALGO_CSDL
+
+
+class ReduceThenScanDispatcher(_Dispatcher):
+    _algo = ALGO_RTS
+
+
+class EmulatedDeadlockingDispatcher(_Dispatcher):
+    _algo = ALGO_CSDLDF

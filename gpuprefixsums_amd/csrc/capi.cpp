@@ -1,0 +1,327 @@
+// C ABI of libgps.so (declared in include/gps.h): handles, argument checks,
+// error reporting and dispatch to the kernel launchers.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/gps.h"
+#include "gps_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(GPS_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+#define HIP_TRY(expr)                                     \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+// Epochs are 30 bits wide; re-arm the status words before a value could recur.
+constexpr uint64_t kLaunchesBeforeReset = (1ull << 30) - 2;
+
+}  // namespace
+
+struct gps_handle {
+    int algo = 0;
+    uint64_t max_n = 0;
+    gps::TileShape shape{256, 4};
+    uint32_t spin_limit = 4;
+    uint32_t stall_period = 0;
+    uint32_t max_tiles = 0;
+    int device = 0;
+    uint64_t* status = nullptr;
+    uint64_t* ticket = nullptr;
+    uint32_t* err = nullptr;
+    unsigned long long* counters = nullptr;
+    uint32_t* tile_sums = nullptr;
+    uint64_t launches = 0;
+};
+
+namespace {
+
+gps::TileShape default_shape(int algo) {
+    switch (algo) {
+        case GPS_ALGO_RTS:
+            return {256, 8};
+        case GPS_ALGO_CSDLDF:
+            return {256, 4};
+        default:
+            return {256, 4};
+    }
+}
+
+void free_handle(gps_handle* h) {
+    if (!h) return;
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(h->device);
+    hipFree(h->status);
+    hipFree(h->ticket);
+    hipFree(h->err);
+    hipFree(h->counters);
+    hipFree(h->tile_sums);
+    hipSetDevice(cur);
+    delete h;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+int check_scan_args(gps_handle* h, const void* in, const void* out, uint64_t n) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer with n=%llu", (unsigned long long)n);
+    if (n > h->max_n)
+        return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
+                    (unsigned long long)h->max_n);
+    if ((reinterpret_cast<uintptr_t>(in) & 3u) || (reinterpret_cast<uintptr_t>(out) & 3u))
+        return fail(GPS_ERR_INVALID, "uint32 buffers must be 4-byte aligned");
+    return GPS_OK;
+}
+
+int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, void* stream) {
+    int rc = check_scan_args(h, in, out, n);
+    if (rc) return rc;
+    if (n == 0) return GPS_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t tile = h->shape.tile();
+    const uint32_t tiles = (uint32_t)((n + tile - 1) / tile);
+    const uint32_t vec_ok = aligned16(in) && aligned16(out);
+    if (h->algo == GPS_ALGO_RTS) {
+        HIP_TRY(gps::launch_rts_u32(h->shape, inclusive, in, out, n, tiles, vec_ok, h->tile_sums, s));
+        return GPS_OK;
+    }
+    if (h->launches >= kLaunchesBeforeReset) {
+        HIP_TRY(gps::launch_reset_status(h->status, h->max_tiles, h->ticket, s));
+        h->launches = 0;
+    }
+    gps::ScanParams p{};
+    p.in = in;
+    p.out = out;
+    p.n = n;
+    p.num_tiles = tiles;
+    p.vec_ok = vec_ok;
+    p.status = h->status;
+    p.ticket = h->ticket;
+    p.err = h->err;
+    p.spin_limit = h->spin_limit;
+    p.stall_period = h->stall_period;
+    p.counters = h->counters;
+    if (h->algo == GPS_ALGO_CSDLDF) {
+        HIP_TRY(gps::launch_csdldf_u32(h->shape, inclusive, p, s));
+    } else {
+        HIP_TRY(gps::launch_csdl_u32(h->shape, inclusive, p, s));
+    }
+    ++h->launches;
+    return GPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gps_version(void) { return "gpuprefixsums_amd 0.1 (gfx950)"; }
+
+const char* gps_last_error(void) { return g_last_error.c_str(); }
+
+int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** out) {
+    if (!out) return fail(GPS_ERR_INVALID, "null output handle pointer");
+    *out = nullptr;
+    if (algo != GPS_ALGO_CSDL && algo != GPS_ALGO_RTS && algo != GPS_ALGO_CSDLDF)
+        return fail(GPS_ERR_INVALID, "unknown algorithm %d", algo);
+    if (max_n == 0) return fail(GPS_ERR_INVALID, "max_n must be positive");
+    gps::TileShape shape = default_shape(algo);
+    uint32_t spin_limit = 4, stall_period = 0;
+    if (tuning) {
+        if (tuning->block_threads) shape.block = tuning->block_threads;
+        if (tuning->vec_per_thread) shape.vec = tuning->vec_per_thread;
+        if (tuning->spin_limit) spin_limit = tuning->spin_limit;
+        stall_period = tuning->stall_period;
+    }
+    if (!gps::shape_supported(shape))
+        return fail(GPS_ERR_INVALID, "unsupported tile shape block=%u vec=%u (block 256/512/1024, vec 2/4/8)",
+                    shape.block, shape.vec);
+    if (stall_period == 1) return fail(GPS_ERR_INVALID, "stall_period must be 0 or >= 2");
+    if (stall_period && algo != GPS_ALGO_CSDLDF)
+        return fail(GPS_ERR_INVALID, "stall emulation needs the decoupled-fallback algorithm");
+    const uint64_t tiles = (max_n + shape.tile() - 1) / shape.tile();
+    if (tiles >= (1ull << 32) - 1) return fail(GPS_ERR_TOO_LARGE, "max_n too large");
+
+    gps_handle* h = new gps_handle();
+    h->algo = algo;
+    h->max_n = max_n;
+    h->shape = shape;
+    h->spin_limit = spin_limit;
+    h->stall_period = stall_period;
+    h->max_tiles = (uint32_t)tiles;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipMalloc(&h->status, tiles * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&h->ticket, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&h->counters, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess && algo == GPS_ALGO_RTS) e = hipMalloc(&h->tile_sums, tiles * sizeof(uint32_t));
+    if (e == hipSuccess) e = gps::launch_reset_status(h->status, tiles, h->ticket, nullptr);
+    if (e == hipSuccess) e = hipMemset(h->err, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(h->counters, 0, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_handle(h);
+        return hip_fail(e, "gps_create");
+    }
+    *out = h;
+    return GPS_OK;
+}
+
+int gps_destroy(gps_handle* h) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    free_handle(h);
+    return GPS_OK;
+}
+
+int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems) {
+    if (!h || !out) return fail(GPS_ERR_INVALID, "null argument");
+    out->block_threads = h->shape.block;
+    out->vec_per_thread = h->shape.vec;
+    out->spin_limit = h->spin_limit;
+    out->stall_period = h->stall_period;
+    if (tile_elems) *tile_elems = h->shape.tile();
+    return GPS_OK;
+}
+
+int gps_inclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream) {
+    return scan_u32(h, d_in, d_out, n, true, stream);
+}
+
+int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream) {
+    return scan_u32(h, d_in, d_out, n, false, stream);
+}
+
+int gps_get_counters(gps_handle* h, uint64_t out[4]) {
+    if (!h || !out) return fail(GPS_ERR_INVALID, "null argument");
+    unsigned long long c[4];
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(c, h->counters, sizeof c, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+    return GPS_OK;
+}
+
+int gps_reset_counters(gps_handle* h) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemset(h->counters, 0, 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipDeviceSynchronize());
+    return GPS_OK;
+}
+
+int gps_device_error(gps_handle* h, uint32_t* bits) {
+    if (!h || !bits) return fail(GPS_ERR_INVALID, "null argument");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(bits, h->err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemset(h->err, 0, sizeof(uint32_t)));
+    HIP_TRY(hipDeviceSynchronize());
+    return GPS_OK;
+}
+
+int gps_reset(gps_handle* h, void* stream) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    HIP_TRY(gps::launch_reset_status(h->status, h->max_tiles, h->ticket, static_cast<hipStream_t>(stream)));
+    h->launches = 0;
+    return GPS_OK;
+}
+
+int gps_time_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive,
+                      void* stream, uint32_t iters, float* ms) {
+    if (!h || !ms || iters == 0) return fail(GPS_ERR_INVALID, "bad timing arguments");
+    int rc = check_scan_args(h, d_in, d_out, n);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<hipEvent_t> ev(iters + 1);
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    int status = GPS_OK;
+    HIP_TRY(hipEventRecord(ev[0], s));
+    for (uint32_t i = 0; i < iters && status == GPS_OK; ++i) {
+        status = scan_u32(h, d_in, d_out, n, inclusive != 0, stream);
+        if (status == GPS_OK) {
+            hipError_t e = hipEventRecord(ev[i + 1], s);
+            if (e != hipSuccess) status = hip_fail(e, "hipEventRecord");
+        }
+    }
+    if (status == GPS_OK) {
+        hipError_t e = hipEventSynchronize(ev[iters]);
+        if (e != hipSuccess) status = hip_fail(e, "hipEventSynchronize");
+        for (uint32_t i = 0; i < iters && status == GPS_OK; ++i) {
+            e = hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+            if (e != hipSuccess) status = hip_fail(e, "hipEventElapsedTime");
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    return status;
+}
+
+int gps_fill_u32(uint32_t* d_ptr, uint64_t n, uint32_t value, void* stream) {
+    if (n && !d_ptr) return fail(GPS_ERR_INVALID, "null pointer");
+    HIP_TRY(gps::launch_fill_u32(d_ptr, n, value, static_cast<hipStream_t>(stream)));
+    return GPS_OK;
+}
+
+int gps_copy_u32(uint32_t* d_dst, const uint32_t* d_src, uint64_t n, void* stream) {
+    if (n && (!d_dst || !d_src)) return fail(GPS_ERR_INVALID, "null pointer");
+    if ((reinterpret_cast<uintptr_t>(d_dst) | reinterpret_cast<uintptr_t>(d_src)) & 3u)
+        return fail(GPS_ERR_INVALID, "uint32 buffers must be 4-byte aligned");
+    HIP_TRY(gps::launch_copy_u32(d_src, d_dst, n, static_cast<hipStream_t>(stream)));
+    return GPS_OK;
+}
+
+int gps_fill_random_u32(uint32_t* d_ptr, uint64_t n, uint64_t seed, uint64_t first, uint32_t bits,
+                        void* stream) {
+    if (n && !d_ptr) return fail(GPS_ERR_INVALID, "null pointer");
+    if (bits < 1 || bits > 32) return fail(GPS_ERR_INVALID, "bits must be in [1, 32]");
+    HIP_TRY(gps::launch_fill_splitmix_u32(d_ptr, n, seed, first, bits, static_cast<hipStream_t>(stream)));
+    return GPS_OK;
+}
+
+static int run_validator(bool ones, const uint32_t* in, const uint32_t* scan, uint64_t n, int inclusive,
+                         void* stream, uint64_t* errors) {
+    if (!errors || (n && (!scan || (!ones && !in)))) return fail(GPS_ERR_INVALID, "null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long* d_err = nullptr;
+    HIP_TRY(hipMalloc(&d_err, sizeof *d_err));
+    hipError_t e = hipMemsetAsync(d_err, 0, sizeof *d_err, s);
+    if (e == hipSuccess)
+        e = ones ? gps::launch_validate_ones_u32(scan, n, inclusive, d_err, s)
+                 : gps::launch_validate_diff_u32(in, scan, n, inclusive, d_err, s);
+    unsigned long long h_err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h_err, d_err, sizeof h_err, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d_err);
+    if (e != hipSuccess) return hip_fail(e, "validator");
+    *errors = h_err;
+    return GPS_OK;
+}
+
+int gps_validate_ones_u32(const uint32_t* d_scan, uint64_t n, int inclusive, void* stream, uint64_t* errors) {
+    return run_validator(true, nullptr, d_scan, n, inclusive, stream, errors);
+}
+
+int gps_validate_scan_u32(const uint32_t* d_in, const uint32_t* d_scan, uint64_t n, int inclusive,
+                          void* stream, uint64_t* errors) {
+    return run_validator(false, d_in, d_scan, n, inclusive, stream, errors);
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+// Device building blocks shared by the gfx950 scan kernels: wave64 DPP scans,
+// 64-bit tile status words with per-launch epochs, and ticketed tile order.
+//
+// Reference counterparts (behaviour, not code):
+//   wave scans      <- GPUPrefixSumsCUDA/Utils.cuh:29-59 (warp-32 shuffle scans)
+//   status flags    <- GPUPrefixSumsCUDA/ChainedScanDecoupledLookback.cuh:19-23
+//   ticketed tiles  <- GPUPrefixSumsCUDA/ChainedScanDecoupledLookback.cuh:89-94
+//   window lookback <- GPUPrefixSumsCUDA/ChainedScanDecoupledLookback.cuh:50-77
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gps {
+
+constexpr uint32_t kWave = 64;
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+__device__ __forceinline__ uint32_t readlane(uint32_t x, int lane) {
+    return __builtin_amdgcn_readlane(x, lane);
+}
+
+// Inclusive scan over the 64 lanes of a wave with DPP: four row_shr steps
+// scan each 16-lane row, then row_bcast:15 and row_bcast:31 carry row totals
+// across rows (GFX9 DPP; each step folds into one v_add_u32_dpp).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
+// Sum over the wave, returned uniformly (lane 63 of the inclusive scan).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) { return readlane(wave_incl_scan(x), 63); }
+
+// ---------------------------------------------------------------------------
+// Tile status words.
+//
+// One 64-bit word per tile, read and written only with relaxed agent-scope
+// atomics (global_load/store_dwordx2 ... sc1 on gfx950: coherent across the
+// eight XCD L2s without the L2 write-back a release would cost). The payload
+// travels inside the word, so no other memory needs ordering:
+//   bits  0..31  value: the tile aggregate (FLAG_AGG) or the inclusive prefix
+//                through this tile (FLAG_INCL), full 32 bits, wraps mod 2^32
+//   bits 32..33  flag
+//   bits 34..63  epoch of the launch that wrote it (30 bits)
+// A word whose epoch differs from the current launch's is "not ready"; the
+// array is therefore never cleared between launches.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg = 1;
+constexpr uint64_t kFlagIncl = 2;
+constexpr uint32_t kEpochBits = 30;
+constexpr uint32_t kEpochMask = (1u << kEpochBits) - 1u;
+
+__device__ __forceinline__ uint64_t make_status(uint32_t value, uint64_t flag, uint32_t epoch) {
+    return (uint64_t)value | (flag << 32) | ((uint64_t)epoch << 34);
+}
+// Flag of a word for this launch's epoch: 0 (not ready), kFlagAgg or kFlagIncl.
+__device__ __forceinline__ uint32_t status_flag(uint64_t w, uint32_t epoch) {
+    return ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
+}
+
+__device__ __forceinline__ uint64_t atomic_load_agent(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_agent(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Publication with a returned value: once the wave has waited on the return,
+// the word is performed at the device coherence point. Used where later
+// output stores must not overtake the publication (in-place fallback safety).
+__device__ __forceinline__ uint64_t atomic_xchg_agent(uint64_t* p, uint64_t v) {
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t atomic_cas_agent(uint64_t* p, uint64_t expected, uint64_t desired) {
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected;  // the value found: == the argument iff the swap happened
+}
+__device__ __forceinline__ void atomic_add_counter(unsigned long long* p, unsigned long long v) {
+    if (p && v) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Ticket word: (epoch << 32) | tickets_taken. Each workgroup takes one ticket
+// with a single 64-bit atomic add, which hands it both its tile index (in
+// dispatch order, so every predecessor tile is resident or finished: the
+// forward-progress argument of decoupled lookback) and the launch epoch. The
+// holder of the last ticket knows every other workgroup has already taken
+// theirs and rearms the word for the next launch: ticket 0, epoch + 1
+// (skipping epochs that are 0 mod 2^30). No memset is needed between launches,
+// so the entry points are one kernel launch and are safe under graph capture.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t next_epoch(uint32_t e) {
+    uint32_t n = e + 1u;
+    if ((n & kEpochMask) == 0u) ++n;
+    return n;
+}
+
+struct Ticket {
+    uint32_t tile;
+    uint32_t epoch;  // already masked to kEpochBits
+};
+
+// Called by one thread.
+__device__ __forceinline__ Ticket take_ticket(uint64_t* ticket_word, uint32_t num_tiles) {
+    const uint64_t old = __hip_atomic_fetch_add(ticket_word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Ticket t;
+    t.tile = (uint32_t)old;
+    const uint32_t e = (uint32_t)(old >> 32);
+    t.epoch = e & kEpochMask;
+    if (t.tile == num_tiles - 1u) atomic_store_agent(ticket_word, (uint64_t)next_epoch(e) << 32);
+    return t;
+}
+
+// Device-side watchdog: a spin that exceeds this many empty polls records an
+// error (read back by the host) and gives up instead of spinning forever.
+// Every 1024 polls a spinner also reads the error word, so once one
+// workgroup has given up the others stop waiting too and the grid drains.
+constexpr uint32_t kWatchdogPolls = 1u << 20;
+
+__device__ __forceinline__ void raise_device_error(uint32_t* err, uint32_t code) {
+    if (err) __hip_atomic_fetch_or(err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Call once per empty poll; true when the spinner must give up.
+__device__ __forceinline__ bool watchdog_expired(uint32_t& polls, const uint32_t* err) {
+    if ((++polls & 1023u) != 0u) return false;
+    if (polls > kWatchdogPolls) return true;
+    return err && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+constexpr uint32_t kErrLookbackWatchdog = 1u;
+constexpr uint32_t kErrStallWatchdog = 2u;
+
+// ---------------------------------------------------------------------------
+// Window lookback (one full wave): returns the exclusive prefix of `tile`,
+// i.e. the sum mod 2^32 of the aggregates of all tiles before it.
+// Lane l reads the word of tile (top - l), top starting at tile - 1. The wave
+// consumes the contiguous ready run nearest to the tile: if that run contains
+// an inclusive word, the prefix is complete; otherwise the run's aggregates
+// are added and the window slides below it. A not-ready nearest word is
+// polled again (s_sleep between polls). Unlike the reference's warp lookback,
+// which waits until all 32 words of its window are ready, the wave never waits
+// on a word it does not need.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lookback_u32(const uint64_t* status, uint32_t tile, uint32_t epoch,
+                                                 uint32_t* err) {
+    const uint32_t lane = lane_id();
+    uint32_t prefix = 0;
+    int64_t top = (int64_t)tile - 1;
+    uint32_t empty_polls = 0;
+    while (true) {
+        const int64_t idx = top - (int64_t)lane;
+        const uint64_t w = idx >= 0 ? atomic_load_agent(status + idx) : make_status(0u, kFlagIncl, epoch);
+        const uint32_t f = status_flag(w, epoch);
+        const uint64_t ready = ballot(f != 0u);
+        const uint64_t incl = ballot(f == (uint32_t)kFlagIncl);
+        const uint32_t run = (~ready) ? (uint32_t)__builtin_ctzll(~ready) : 64u;
+        const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+        if (first_incl < run) {
+            prefix += wave_sum(lane <= first_incl ? (uint32_t)w : 0u);
+            return prefix;
+        }
+        if (run) {
+            prefix += wave_sum(lane < run ? (uint32_t)w : 0u);
+            top -= run;
+            empty_polls = 0;
+        } else {
+            if (watchdog_expired(empty_polls, err)) {
+                if (lane == 0) raise_device_error(err, kErrLookbackWatchdog);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+}  // namespace gps

@@ -1,0 +1,55 @@
+// Internal (host-side) interface between the C ABI layer and the kernel
+// launchers. Not installed; the public surface is include/gps.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gps {
+
+// Launch parameters shared by the single-pass uint32 kernels (CSDL, CSDLDF).
+struct ScanParams {
+    const uint32_t* in;
+    uint32_t* out;
+    uint64_t n;
+    uint32_t num_tiles;
+    uint32_t vec_ok;  // in and out are 16-byte aligned
+    uint64_t* status;  // one word per tile
+    uint64_t* ticket;  // (epoch << 32) | tickets taken
+    uint32_t* err;     // device error bits (watchdogs)
+    // decoupled fallback only
+    uint32_t spin_limit;
+    uint32_t stall_period;              // 0: no emulated stalls
+    unsigned long long* counters;       // [polls, fallbacks, insertions, stall timeouts] or null
+};
+
+struct TileShape {
+    uint32_t block;
+    uint32_t vec;
+    uint32_t tile() const { return block * vec * 4u; }
+};
+
+bool shape_supported(TileShape s);
+
+// Each launcher enqueues exactly one kernel (RTS: three) on `stream` and
+// returns the HIP launch status.
+hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
+hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
+hipError_t launch_rts_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
+                          uint32_t num_tiles, uint32_t vec_ok, uint32_t* tile_sums, hipStream_t stream);
+
+// Clears `count` status words and arms the ticket word at epoch 1.
+hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream);
+
+// Utility kernels (the reference's InitOne / Validate* family and a
+// counter-based random fill).
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t stream);
+hipError_t launch_copy_u32(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t stream);
+hipError_t launch_fill_splitmix_u32(uint32_t* p, uint64_t n, uint64_t seed, uint64_t first, uint32_t bits,
+                                    hipStream_t stream);
+hipError_t launch_validate_ones_u32(const uint32_t* scan, uint64_t n, int inclusive,
+                                    unsigned long long* errors, hipStream_t stream);
+hipError_t launch_validate_diff_u32(const uint32_t* in, const uint32_t* scan, uint64_t n, int inclusive,
+                                    unsigned long long* errors, hipStream_t stream);
+
+}  // namespace gps

@@ -1,0 +1,540 @@
+// uint32 device-wide scans for gfx950:
+//   csdl_u32    chained scan with decoupled lookback (single pass, 8 B/elem)
+//   csdldf_u32  the same with decoupled fallback: bounded spins, a
+//               workgroup-wide redundant reduction of the blocking tile and
+//               CAS insertion; optional emulated deadlock (stalled tiles)
+//   rts_*_u32   reduce-then-scan (reduce, single-workgroup spine, downsweep)
+//
+// Reference counterparts:
+//   GPUPrefixSumsCUDA/ChainedScanDecoupledLookback.cuh:79-153 (CSDL)
+//   GPUPrefixSumsCUDA/EmulatedDeadlocking.cuh:26-168 (fallback + emulation)
+//   GPUPrefixSumsD3D12/Shaders/ChainedScanDecoupledLookbackDecoupledFallback.hlsl:81-242
+//     (the inclusive and exclusive fallback kernels)
+//   GPUPrefixSumsCUDA/ReduceThenScan.cuh:15-161 (RTS)
+#include "gps_internal.hpp"
+#include "tile_u32.hpp"
+
+namespace gps {
+
+// ---------------------------------------------------------------------------
+// Shared front half: take a ticket, load the tile, scan it in registers and
+// combine the wave runs. Leaves d scanned (without the tile prefix) and sets
+// wave_prefix / tile_total.
+// ---------------------------------------------------------------------------
+template <int BLOCK, int VEC, bool INCL>
+struct TileFront {
+    using C = TileCfg<BLOCK, VEC>;
+    u32x4 d[VEC];
+    uint32_t tile, epoch, wave_prefix, tile_total;
+    uint64_t base;
+    bool vec_path;
+
+    __device__ __forceinline__ void run(const ScanParams& p, uint32_t* s_wave, uint32_t* s_ticket) {
+        const uint32_t wave = threadIdx.x / kWave;
+        if (threadIdx.x == 0) {
+            const Ticket t = take_ticket(p.ticket, p.num_tiles);
+            s_ticket[0] = t.tile;
+            s_ticket[1] = t.epoch;
+        }
+        __syncthreads();
+        tile = s_ticket[0];
+        epoch = s_ticket[1];
+        base = thread_base<BLOCK, VEC>(tile);
+        vec_path = p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n);
+        if (vec_path)
+            load_vec<VEC>(d, p.in + base);
+        else
+            load_guarded<VEC>(d, p.in, base, p.n);
+        const uint32_t run_total = wave_scan_run<VEC, INCL>(d);
+        if (lane_id() == 0) s_wave[wave] = run_total;
+        __syncthreads();
+        combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
+    }
+
+    __device__ __forceinline__ void store(const ScanParams& p, uint32_t add) {
+        if (vec_path)
+            store_vec<VEC>(p.out + base, d, add);
+        else
+            store_guarded<VEC>(p.out, d, add, base, p.n);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// CSDL. Wave 0 publishes the tile aggregate, looks back, and publishes the
+// inclusive prefix; every thread then adds tile prefix + wave prefix.
+// ---------------------------------------------------------------------------
+template <int BLOCK, int VEC, bool INCL>
+__global__ __launch_bounds__(BLOCK) void csdl_u32(ScanParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ uint32_t s_wave[C::kWaves];
+    __shared__ uint32_t s_ticket[2];
+    __shared__ uint32_t s_prefix;
+
+    TileFront<BLOCK, VEC, INCL> f;
+    f.run(p, s_wave, s_ticket);
+
+    if (threadIdx.x < kWave) {
+        const uint32_t lane = lane_id();
+        if (f.tile == 0) {
+            if (lane == 0) {
+                atomic_store_agent(p.status, make_status(f.tile_total, kFlagIncl, f.epoch));
+                s_prefix = 0;
+            }
+        } else {
+            if (lane == 0) atomic_store_agent(p.status + f.tile, make_status(f.tile_total, kFlagAgg, f.epoch));
+            const uint32_t prefix = lookback_u32(p.status, f.tile, f.epoch, p.err);
+            if (lane == 0) {
+                atomic_store_agent(p.status + f.tile, make_status(prefix + f.tile_total, kFlagIncl, f.epoch));
+                s_prefix = prefix;
+            }
+        }
+    }
+    __syncthreads();
+    f.store(p, s_prefix + f.wave_prefix);
+}
+
+// ---------------------------------------------------------------------------
+// CSDLDF: lookback with decoupled fallback.
+//
+// Wave 0 polls the status window; after spin_limit consecutive polls without
+// progress it hands the nearest not-ready tile index j to the workgroup, which
+// reduces tile j's input from memory. Lane 0 then CAS-inserts that aggregate
+// into j's word (inclusive if j == 0) if the word is still stale; on a lost
+// race it uses the word it found. Either way the lookback continues below j,
+// or ends at an inclusive word.
+//
+// Emulated deadlock: with stall_period > 0, tiles with tile % period == 0 that
+// have a successor hold back their own publication until some successor has
+// inserted their aggregate (bounded by a watchdog). The reference stalls
+// !(idx & 7) (period 8) and also stalls a last tile, which would never be
+// released; here the last tile never stalls.
+//
+// In-place safety (in == out): a fallback reads tile j's input. Its CAS only
+// wins while j has not published, and j publishes with a returning exchange
+// that it waits on before its first output store, so a winning reduction
+// never saw j's output.
+// ---------------------------------------------------------------------------
+
+template <int BLOCK, int VEC, bool INCL>
+__global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ uint32_t s_wave[C::kWaves];
+    __shared__ uint32_t s_red[C::kWaves];
+    __shared__ uint32_t s_ticket[2];
+    __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_poll_done, s_cas_done, s_fb_tile;
+
+    TileFront<BLOCK, VEC, INCL> f;
+    f.run(p, s_wave, s_ticket);
+    const uint32_t tile = f.tile, epoch = f.epoch;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+
+    if (threadIdx.x == 0) {
+        unsigned long long stall_timeouts = 0;
+        if (p.stall_period && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
+            uint32_t polls = 0;
+            while (status_flag(atomic_load_agent(p.status + tile), epoch) == 0u) {
+                if (watchdog_expired(polls, p.err)) {
+                    stall_timeouts = 1;
+                    raise_device_error(p.err, kErrStallWatchdog);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        atomic_xchg_agent(p.status + tile, make_status(f.tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+        if (p.counters) atomic_add_counter(p.counters + 3, stall_timeouts);
+        if (tile == 0) s_prefix = 0;
+    }
+
+    if (tile != 0) {
+        // Lookback state, meaningful in wave 0 only (uniform there).
+        uint32_t prefix = 0, spins = 0;
+        int64_t top = (int64_t)tile - 1;
+        uint64_t stale = 0;
+        unsigned long long c_polls = 0, c_fallbacks = 0, c_inserts = 0;
+
+        while (true) {
+            if (wave == 0) {
+                uint32_t done = 0;
+                while (true) {
+                    const int64_t idx = top - (int64_t)lane;
+                    const uint64_t w = idx >= 0 ? atomic_load_agent(p.status + idx) : make_status(0u, kFlagIncl, epoch);
+                    const uint32_t fl = status_flag(w, epoch);
+                    const uint64_t ready = ballot(fl != 0u);
+                    const uint64_t incl = ballot(fl == (uint32_t)kFlagIncl);
+                    const uint32_t run = (~ready) ? (uint32_t)__builtin_ctzll(~ready) : 64u;
+                    const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+                    ++c_polls;
+                    if (first_incl < run) {
+                        prefix += wave_sum(lane <= first_incl ? (uint32_t)w : 0u);
+                        done = 1;
+                        break;
+                    }
+                    if (run) {
+                        prefix += wave_sum(lane < run ? (uint32_t)w : 0u);
+                        top -= run;
+                        spins = 0;
+                    } else if (++spins >= p.spin_limit) {
+                        stale = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
+                        break;
+                    } else {
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                if (lane == 0) {
+                    if (done) {
+                        atomic_xchg_agent(p.status + tile, make_status(prefix + f.tile_total, kFlagIncl, epoch));
+                        s_prefix = prefix;
+                    } else {
+                        s_fb_tile = (uint32_t)top;
+                    }
+                    s_poll_done = done;
+                }
+            }
+            __syncthreads();
+            if (s_poll_done) break;
+
+            // Workgroup-wide redundant reduction of tile j (always a full tile:
+            // only the last tile is partial and it is nobody's predecessor).
+            const uint32_t j = s_fb_tile;
+            {
+                u32x4 fd[VEC];
+                const uint64_t fbase = thread_base<BLOCK, VEC>(j);
+                if (p.vec_ok)
+                    load_vec<VEC>(fd, p.in + fbase);
+                else
+                    load_guarded<VEC>(fd, p.in, fbase, p.n);
+                const uint32_t ws = wave_sum(thread_sum<VEC>(fd));
+                if (lane == 0) s_red[wave] = ws;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                uint32_t fsum = 0;
+#pragma unroll
+                for (uint32_t q = 0; q < C::kWaves; ++q) fsum += s_red[q];
+                const uint64_t mine = make_status(fsum, j ? kFlagAgg : kFlagIncl, epoch);
+                uint64_t found = mine;
+                if (lane == 0) found = atomic_cas_agent(p.status + j, stale, mine);
+                found = ((uint64_t)readlane((uint32_t)(found >> 32), 0) << 32) | readlane((uint32_t)found, 0);
+                ++c_fallbacks;
+                uint32_t fl;
+                if (found == stale) {  // inserted ours
+                    ++c_inserts;
+                    fl = j ? (uint32_t)kFlagAgg : (uint32_t)kFlagIncl;
+                    found = mine;
+                } else {
+                    fl = status_flag(found, epoch);
+                }
+                uint32_t done = 0;
+                if (fl != 0u) {
+                    prefix += (uint32_t)found;
+                    if (fl == (uint32_t)kFlagIncl) {
+                        done = 1;
+                    } else {
+                        top = (int64_t)j - 1;
+                    }
+                } else {
+                    // A lost CAS always returns a current-epoch word (the only
+                    // in-launch writes); anything else is a protocol fault.
+                    if (lane == 0) raise_device_error(p.err, kErrLookbackWatchdog);
+                    done = 1;
+                }
+                spins = 0;
+                if (lane == 0) {
+                    if (done) {
+                        atomic_xchg_agent(p.status + tile, make_status(prefix + f.tile_total, kFlagIncl, epoch));
+                        s_prefix = prefix;
+                    }
+                    s_cas_done = done;
+                }
+            }
+            __syncthreads();
+            if (s_cas_done) break;
+        }
+        if (threadIdx.x == 0 && p.counters) {
+            atomic_add_counter(p.counters + 0, c_polls);
+            atomic_add_counter(p.counters + 1, c_fallbacks);
+            atomic_add_counter(p.counters + 2, c_inserts);
+        }
+    }
+    __syncthreads();
+    f.store(p, s_prefix + f.wave_prefix);
+}
+
+// ---------------------------------------------------------------------------
+// Reduce-then-scan. Three launches: per-tile sums, an exclusive scan of the
+// sums by one workgroup, and a rescan of every tile with its carry-in. Reads
+// the input twice (12 B/elem of traffic against 8 B algorithmic).
+// The reference's partial-tile reduce predicate is inverted
+// (ReduceThenScan.cuh:36-37); here the last tile is reduced over [.., n).
+// ---------------------------------------------------------------------------
+template <int BLOCK, int VEC>
+__global__ __launch_bounds__(BLOCK) void rts_reduce_u32(const uint32_t* in, uint64_t n, uint32_t vec_ok,
+                                                        uint32_t* tile_sums) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ uint32_t s_red[C::kWaves];
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    u32x4 d[VEC];
+    if (vec_ok && tile_is_full<BLOCK, VEC>(tile, n))
+        load_vec<VEC>(d, in + base);
+    else
+        load_guarded<VEC>(d, in, base, n);
+    const uint32_t ws = wave_sum(thread_sum<VEC>(d));
+    if (lane_id() == 0) s_red[threadIdx.x / kWave] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < C::kWaves; ++q) t += s_red[q];
+        tile_sums[tile] = t;
+    }
+}
+
+// One workgroup of 1024 threads; exclusive scan of tile_sums in place, 4096
+// values per round with a uniform carry between rounds.
+constexpr int kSpineBlock = 1024;
+__global__ __launch_bounds__(kSpineBlock) void rts_spine_u32(uint32_t* tile_sums, uint32_t num_tiles) {
+    using C = TileCfg<kSpineBlock, 1>;
+    __shared__ uint32_t s_wave[C::kWaves];
+    const uint32_t wave = threadIdx.x / kWave;
+    uint32_t carry = 0;
+    for (uint32_t round = 0; round * C::kTile < num_tiles; ++round) {
+        const uint64_t base = thread_base<kSpineBlock, 1>(round);
+        u32x4 d[1];
+        load_guarded<1>(d, tile_sums, base, num_tiles);
+        const uint32_t run = wave_scan_run<1, false>(d);
+        if (lane_id() == 0) s_wave[wave] = run;
+        __syncthreads();
+        uint32_t wave_prefix, total;
+        combine_waves<C::kWaves>(s_wave, wave, wave_prefix, total);
+        store_guarded<1>(tile_sums, d, carry + wave_prefix, base, num_tiles);
+        carry += total;
+        __syncthreads();
+    }
+}
+
+template <int BLOCK, int VEC, bool INCL>
+__global__ __launch_bounds__(BLOCK) void rts_downsweep_u32(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                           uint32_t vec_ok, const uint32_t* tile_prefix) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ uint32_t s_wave[C::kWaves];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    const bool vec_path = vec_ok && tile_is_full<BLOCK, VEC>(tile, n);
+    u32x4 d[VEC];
+    if (vec_path)
+        load_vec<VEC>(d, in + base);
+    else
+        load_guarded<VEC>(d, in, base, n);
+    const uint32_t run = wave_scan_run<VEC, INCL>(d);
+    if (lane_id() == 0) s_wave[wave] = run;
+    __syncthreads();
+    uint32_t wave_prefix, total;
+    combine_waves<C::kWaves>(s_wave, wave, wave_prefix, total);
+    const uint32_t add = tile_prefix[tile] + wave_prefix;
+    if (vec_path)
+        store_vec<VEC>(out + base, d, add);
+    else
+        store_guarded<VEC>(out, d, add, base, n);
+}
+
+// ---------------------------------------------------------------------------
+// Utility kernels.
+// ---------------------------------------------------------------------------
+__global__ void reset_status_kernel(uint64_t* status, uint64_t count, uint64_t* ticket) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) status[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 1ull << 32;
+}
+
+// UtilityKernels.cuh:15-20 (InitOne), generalised to any value.
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+}
+
+// Counter-based splitmix64 (same stream definition as the host generators):
+// element i = top `bits` bits of mix(seed + (first + i + 1) * golden gamma).
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void fill_splitmix_u32_kernel(uint32_t* p, uint64_t n, uint64_t seed, uint64_t first, uint32_t shift) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = (uint32_t)(splitmix64_at(seed, first + i) >> shift);
+}
+
+// UtilityKernels.cuh:22-38: count scan[i] != i+1 (inclusive) / != i (exclusive).
+__global__ void validate_ones_kernel(const uint32_t* scan, uint64_t n, uint32_t incl,
+                                     unsigned long long* errors) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long e = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        e += scan[i] != (uint32_t)i + incl;
+    atomic_add_counter(errors, e);
+}
+
+// Parallel form of the serial validator (Utility.hlsl:99-125): a sequence is
+// the inclusive scan of x iff out[0] == x[0] and out[i] - out[i-1] == x[i]
+// (mod 2^32) for i > 0; exclusive: out[0] == 0 and out[i] - out[i-1] == x[i-1].
+__global__ void validate_diff_kernel(const uint32_t* in, const uint32_t* scan, uint64_t n, uint32_t incl,
+                                     unsigned long long* errors) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long e = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t prev = i ? scan[i - 1] : 0u;
+        const uint32_t x = incl ? in[i] : (i ? in[i - 1] : 0u);
+        e += (uint32_t)(scan[i] - prev) != x;
+    }
+    atomic_add_counter(errors, e);
+}
+
+// Streaming copy with the scan's tile geometry and access pattern (dwordx4,
+// nontemporal): the practical read-n + write-n ceiling the scan is held to,
+// and the calibration kernel for HBM byte counters.
+template <int BLOCK, int VEC>
+__global__ __launch_bounds__(BLOCK) void copy_u32_kernel(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                         uint32_t vec_ok) {
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    u32x4 d[VEC];
+    if (vec_ok && tile_is_full<BLOCK, VEC>(tile, n)) {
+        load_vec<VEC>(d, in + base);
+        store_vec<VEC>(out + base, d, 0u);
+    } else {
+        load_guarded<VEC>(d, in, base, n);
+        store_guarded<VEC>(out, d, 0u, base, n);
+    }
+}
+
+hipError_t launch_copy_u32(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    constexpr uint32_t kTile = TileCfg<256, 4>::kTile;
+    const uint32_t vec_ok = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    hipLaunchKernelGGL((copy_u32_kernel<256, 4>), dim3((uint32_t)((n + kTile - 1) / kTile)), dim3(256), 0, stream,
+                       in, out, n, vec_ok);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Launchers: runtime tile shape -> template instance.
+// ---------------------------------------------------------------------------
+#define GPS_SHAPES(X) \
+    X(256, 2)         \
+    X(256, 4)         \
+    X(256, 8)         \
+    X(512, 2)         \
+    X(512, 4)         \
+    X(512, 8)         \
+    X(1024, 2)        \
+    X(1024, 4)        \
+    X(1024, 8)
+
+bool shape_supported(TileShape s) {
+#define X(B, V) \
+    if (s.block == B && s.vec == V) return true;
+    GPS_SHAPES(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+#define X(B, V)                                                                          \
+    if (s.block == B && s.vec == V) {                                                    \
+        if (inclusive)                                                                   \
+            hipLaunchKernelGGL((csdl_u32<B, V, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);  \
+        else                                                                             \
+            hipLaunchKernelGGL((csdl_u32<B, V, false>), dim3(p.num_tiles), dim3(B), 0, stream, p); \
+        return hipGetLastError();                                                        \
+    }
+    GPS_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+#define X(B, V)                                                                            \
+    if (s.block == B && s.vec == V) {                                                      \
+        if (inclusive)                                                                     \
+            hipLaunchKernelGGL((csdldf_u32<B, V, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);  \
+        else                                                                               \
+            hipLaunchKernelGGL((csdldf_u32<B, V, false>), dim3(p.num_tiles), dim3(B), 0, stream, p); \
+        return hipGetLastError();                                                          \
+    }
+    GPS_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rts_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
+                          uint32_t num_tiles, uint32_t vec_ok, uint32_t* tile_sums, hipStream_t stream) {
+#define X(B, V)                                                                                           \
+    if (s.block == B && s.vec == V) {                                                                     \
+        hipLaunchKernelGGL((rts_reduce_u32<B, V>), dim3(num_tiles), dim3(B), 0, stream, in, n, vec_ok,    \
+                           tile_sums);                                                                    \
+        hipLaunchKernelGGL(rts_spine_u32, dim3(1), dim3(kSpineBlock), 0, stream, tile_sums, num_tiles);  \
+        if (inclusive)                                                                                    \
+            hipLaunchKernelGGL((rts_downsweep_u32<B, V, true>), dim3(num_tiles), dim3(B), 0, stream, in,  \
+                               out, n, vec_ok, (const uint32_t*)tile_sums);                               \
+        else                                                                                              \
+            hipLaunchKernelGGL((rts_downsweep_u32<B, V, false>), dim3(num_tiles), dim3(B), 0, stream, in, \
+                               out, n, vec_ok, (const uint32_t*)tile_sums);                               \
+        return hipGetLastError();                                                                         \
+    }
+    GPS_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+static inline dim3 grid_for(uint64_t n, uint32_t block) {
+    uint64_t g = (n + block - 1) / block;
+    if (g > 4096) g = 4096;
+    if (g == 0) g = 1;
+    return dim3((uint32_t)g);
+}
+
+hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_status_kernel, grid_for(count, 256), dim3(256), 0, stream, status, count, ticket);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, grid_for(n, 256), dim3(256), 0, stream, p, n, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_splitmix_u32(uint32_t* p, uint64_t n, uint64_t seed, uint64_t first, uint32_t bits,
+                                    hipStream_t stream) {
+    if (!n) return hipSuccess;
+    if (bits < 1 || bits > 32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fill_splitmix_u32_kernel, grid_for(n, 256), dim3(256), 0, stream, p, n, seed, first,
+                       64u - bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate_ones_u32(const uint32_t* scan, uint64_t n, int inclusive,
+                                    unsigned long long* errors, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(validate_ones_kernel, grid_for(n, 256), dim3(256), 0, stream, scan, n,
+                       inclusive ? 1u : 0u, errors);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate_diff_u32(const uint32_t* in, const uint32_t* scan, uint64_t n, int inclusive,
+                                    unsigned long long* errors, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(validate_diff_kernel, grid_for(n, 256), dim3(256), 0, stream, in, scan, n,
+                       inclusive ? 1u : 0u, errors);
+    return hipGetLastError();
+}
+
+}  // namespace gps

@@ -1,0 +1,130 @@
+/*
+ * gps.h - C ABI of the MI355X (gfx950) prefix-sum library libgps.so.
+ *
+ * Plain pointers and sizes only: device pointers are HIP device addresses,
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream). Every
+ * function returns a gps_status; gps_last_error() gives the message of the
+ * calling thread's last failure. Nothing prints.
+ *
+ * Reference interfaces replaced (file:line in b0nes164/GPUPrefixSums):
+ *   The reference's scan entry points are the private dispatcher methods
+ *   DispatchKernelsInclusive / DispatchKernelsExclusive(uint32_t size)
+ *     GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:177-195
+ *     GPUPrefixSumsCUDA/ReduceThenScanDispatcher.cuh:171-187
+ *     GPUPrefixSumsCUDA/EmulatedDeadlockingDispatcher.cuh:159-166
+ *   which scan the dispatcher-owned buffer in place after clearing the status
+ *   words and the tile counter (ClearMemory, ...Dispatcher.cuh:172-175). Here
+ *   the handle owns that scratch, the caller owns the data (in == out gives the
+ *   reference's in-place behaviour), and no clear is needed per call.
+ */
+#ifndef GPS_H
+#define GPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gps_status {
+    GPS_OK = 0,
+    GPS_ERR_INVALID = 1,     /* bad argument (null pointer, bad tuning, ...) */
+    GPS_ERR_TOO_LARGE = 2,   /* n exceeds the handle's max_n */
+    GPS_ERR_HIP = 3,         /* a HIP runtime call failed */
+    GPS_ERR_UNSUPPORTED = 4, /* operation not offered by this handle's algorithm */
+    GPS_ERR_DEVICE = 5       /* a device-side watchdog fired (see gps_device_error) */
+} gps_status;
+
+typedef enum gps_algo {
+    GPS_ALGO_CSDL = 0,   /* chained scan with decoupled lookback
+                            (ChainedScanDecoupledLookback.cuh:79-153) */
+    GPS_ALGO_RTS = 1,    /* reduce-then-scan (ReduceThenScan.cuh:15-161) */
+    GPS_ALGO_CSDLDF = 2  /* decoupled lookback with decoupled fallback, optional
+                            emulated deadlock (EmulatedDeadlocking.cuh:26-168;
+                            ...DecoupledFallback.hlsl:81-242) */
+} gps_algo;
+
+/* Runtime tuning; zero fields take defaults. Reference knobs: CSDL_WARPS /
+ * UINT4_PER_THREAD (ChainedScanDecoupledLookbackDispatcher.cuh:13-14),
+ * MAX_SPIN_COUNT / DEADLOCK_MASK (EmulatedDeadlocking.cuh:15-16). */
+typedef struct gps_tuning {
+    uint32_t block_threads;  /* 256, 512 or 1024 (whole wave64s) */
+    uint32_t vec_per_thread; /* uint4 vectors per thread: 2, 4 or 8 (items = 4x) */
+    uint32_t spin_limit;     /* CSDLDF: polls without progress before a fallback (default 4) */
+    uint32_t stall_period;   /* CSDLDF: stall tiles with index % period == 0 (0 = no emulation, >= 2) */
+} gps_tuning;
+
+typedef struct gps_handle gps_handle;
+
+/* Library version string. */
+const char* gps_version(void);
+/* Message of this thread's last failed call ("" if none). */
+const char* gps_last_error(void);
+
+/* Creates a handle that owns the scratch for scans of up to max_n elements on
+ * the current HIP device (ChainedScanDecoupledLookbackDispatcher.cuh:28-34).
+ * tuning may be NULL. */
+int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** out);
+/* (...Dispatcher.cuh:158-163) */
+int gps_destroy(gps_handle* h);
+/* Effective tuning and tile size (elements per workgroup) of a handle. */
+int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems);
+
+/* uint32 scans, wraparound mod 2^32, bit-exact with the serial loop of
+ * GPUPrefixSumsD3D12/Shaders/Utility.hlsl:99-125. d_in may equal d_out.
+ * Enqueued on `stream`; returns after the launch, not the completion.
+ * Replaces DispatchKernelsInclusive (...Dispatcher.cuh:187-195) and
+ * DispatchKernelsExclusive (...Dispatcher.cuh:177-185). */
+int gps_inclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
+int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
+
+/* CSDLDF fallback counters accumulated since the last reset:
+ * [0] status polls, [1] fallbacks (redundant tile reductions), [2] successful
+ * CAS insertions, [3] emulated stalls that hit their watchdog.
+ * Synchronises the handle's device. Cf. TestVariants/csdldf_stats.wgsl:285-290. */
+int gps_get_counters(gps_handle* h, uint64_t out[4]);
+int gps_reset_counters(gps_handle* h);
+/* Device error bits raised by watchdogs since the last call (and clears
+ * them): 1 = a lookback exceeded its poll budget, 2 = an emulated stall did.
+ * Synchronises the device. */
+int gps_device_error(gps_handle* h, uint32_t* bits);
+/* Re-arms the status words (epoch wrap after 2^30-1 launches; the direct entry
+ * points do this themselves, graph replays must call it). */
+int gps_reset(gps_handle* h, void* stream);
+
+/* Times `iters` back-to-back calls of the handle's scan on `stream` with a
+ * hipEvent pair around each call; writes per-call milliseconds to ms[iters].
+ * Cf. BatchTimingInclusive (...Dispatcher.cuh:116-156). */
+int gps_time_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive,
+                      void* stream, uint32_t iters, float* ms);
+
+/* ---- utility kernels (UtilityKernels.cuh:15-38 and a random fill) -------- */
+
+int gps_fill_u32(uint32_t* d_ptr, uint64_t n, uint32_t value, void* stream);
+/* Streaming copy with the scans' tile geometry (read n + write n): the
+ * practical bandwidth ceiling the scans are measured against. No reference
+ * counterpart. */
+int gps_copy_u32(uint32_t* d_dst, const uint32_t* d_src, uint64_t n, void* stream);
+/* d_ptr[i] = top `bits` bits of splitmix64(seed, first + i) (the same stream
+ * the CPU oracle generates). */
+int gps_fill_random_u32(uint32_t* d_ptr, uint64_t n, uint64_t seed, uint64_t first, uint32_t bits,
+                        void* stream);
+/* Counts mismatches on the device and returns them (synchronises `stream`):
+ * all-ones closed form (UtilityKernels.cuh:22-38) and the parallel form of the
+ * serial validator (Utility.hlsl:99-125). */
+int gps_validate_ones_u32(const uint32_t* d_scan, uint64_t n, int inclusive, void* stream, uint64_t* errors);
+int gps_validate_scan_u32(const uint32_t* d_in, const uint32_t* d_scan, uint64_t n, int inclusive,
+                          void* stream, uint64_t* errors);
+
+/* ---- reference dispatcher self-tests (C++ classes in gps_dispatchers.hpp) - */
+
+/* Runs <Dispatcher>::TestAllInclusive / TestAllExclusive for algo with a
+ * dispatcher of max_size elements; writes tests passed and tests run.
+ * (ChainedScanDecoupledLookbackDispatcher.cuh:38-114) */
+int gps_dispatcher_test_all(int algo, uint32_t max_size, int inclusive, uint32_t* passed, uint32_t* total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPS_H */

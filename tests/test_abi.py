@@ -1,0 +1,65 @@
+"""The C-ABI library loads and exports everything include/*.h declares.
+
+CPU only: no call here touches a GPU (only argument checks that fail before
+any HIP call are exercised).
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions() -> set:
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"\b(gps_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for required in ("gps_create", "gps_destroy", "gps_inclusive_scan_u32", "gps_exclusive_scan_u32"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(gps_lib):
+    missing = [n for n in sorted(declared_functions()) if not hasattr(gps_lib, n)]
+    assert not missing, f"libgps.so lacks {missing}"
+
+
+def test_python_binding_covers_header():
+    from gpuprefixsums_amd import _lib
+    assert declared_functions() == set(_lib.SIGNATURES), (
+        declared_functions() ^ set(_lib.SIGNATURES))
+
+
+def test_argument_errors_before_any_device_call(gps_lib):
+    from gpuprefixsums_amd._lib import GPS_ERR_INVALID, GpsTuning
+    h = ctypes.c_void_p()
+    assert gps_lib.gps_create(0, 1024, None, None) == GPS_ERR_INVALID
+    assert gps_lib.gps_create(7, 1024, None, ctypes.byref(h)) == GPS_ERR_INVALID
+    assert b"unknown algorithm" in gps_lib.gps_last_error()
+    assert gps_lib.gps_create(0, 0, None, ctypes.byref(h)) == GPS_ERR_INVALID
+    bad = GpsTuning(384, 4, 0, 0)
+    assert gps_lib.gps_create(0, 1024, ctypes.byref(bad), ctypes.byref(h)) == GPS_ERR_INVALID
+    assert b"unsupported tile shape" in gps_lib.gps_last_error()
+    stall_on_csdl = GpsTuning(0, 0, 0, 8)
+    assert gps_lib.gps_create(0, 1024, ctypes.byref(stall_on_csdl), ctypes.byref(h)) == GPS_ERR_INVALID
+    period_one = GpsTuning(0, 0, 0, 1)
+    assert gps_lib.gps_create(2, 1024, ctypes.byref(period_one), ctypes.byref(h)) == GPS_ERR_INVALID
+    assert gps_lib.gps_inclusive_scan_u32(None, None, None, 0, None) == GPS_ERR_INVALID
+    assert gps_lib.gps_destroy(None) == GPS_ERR_INVALID
+    assert gps_lib.gps_version().startswith(b"gpuprefixsums_amd")
+
+
+def test_binding_fails_loudly_without_library(tmp_path, monkeypatch):
+    from gpuprefixsums_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "missing.so")
+    with pytest.raises(ImportError):
+        _lib.load()

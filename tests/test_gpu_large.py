@@ -1,0 +1,71 @@
+"""n = 2^28 full-range uint32 scans (1 GiB in, 1 GiB out), bit-exact against
+the CPU oracle over every element: CSDL, reduce-then-scan, and the decoupled
+fallback under emulated deadlock with stall periods 7 (every 7th tile) and 8
+(the reference's !(idx & 7), GPUPrefixSumsCUDA/EmulatedDeadlocking.cuh:16,65).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import gpuprefixsums_amd as gps  # noqa: E402
+from tests.gpu_util import empty_u32, first_mismatch, from_dev_u32, sync, to_dev  # noqa: E402
+
+N = 1 << 28
+
+
+@pytest.fixture(scope="module")
+def data():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+    x = oracle.gen_u32(2, N, 32)
+    d_in = to_dev(x)
+    d_out = empty_u32(N)
+    return x, d_in, d_out
+
+
+@pytest.fixture(scope="module")
+def want_incl(data):
+    return oracle.scan_u32(data[0], True)
+
+
+CASES = [
+    ("csdl", dict(algo=gps.ALGO_CSDL)),
+    ("rts", dict(algo=gps.ALGO_RTS)),
+    ("csdldf_stall7", dict(algo=gps.ALGO_CSDLDF, stall_period=7)),
+    ("csdldf_stall8", dict(algo=gps.ALGO_CSDLDF, stall_period=8)),
+]
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_inclusive_2p28_full_range(data, want_incl, name, kw):
+    x, d_in, d_out = data
+    with gps.ScanHandle(max_n=N, **kw) as h:
+        d_out.fill_(0)
+        h.inclusive(d_in, d_out, N)
+        sync()
+        got = from_dev_u32(d_out)
+        assert np.array_equal(got, want_incl), f"{name}: {first_mismatch(got, want_incl)}"
+        # second launch on the same handle (no clearing in between)
+        h.inclusive(d_in, d_out, N)
+        sync()
+        assert np.array_equal(from_dev_u32(d_out), want_incl)
+        if kw.get("stall_period"):
+            c = h.counters()
+            assert c["stall_timeouts"] == 0 and c["insertions"] > 0
+        assert h.device_error() == 0
+
+
+def test_exclusive_2p28_full_range(data):
+    x, d_in, d_out = data
+    want = oracle.scan_u32(x, False)
+    for kw in (dict(algo=gps.ALGO_CSDL), dict(algo=gps.ALGO_CSDLDF, stall_period=7)):
+        with gps.ScanHandle(max_n=N, **kw) as h:
+            h.exclusive(d_in, d_out, N)
+            sync()
+            got = from_dev_u32(d_out)
+            assert np.array_equal(got, want), first_mismatch(got, want)

@@ -1,0 +1,254 @@
+"""uint32 scans on the GPU, bit-exact against the CPU oracle (through the C ABI).
+
+Covers the reference's own tests (all-ones sweeps over one to two tiles,
+GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:38-114; random
+[0,1023] inputs with seed = n over n in [3072, 6144),
+GPUPrefixSumsD3D12/GPUPrefixSumBase.h:135-161) and what they leave out:
+full-range wraparound, exclusive decoupled fallback, stall periods 7 and 8,
+n % 4 tails, unaligned and in-place buffers, every tile shape, 2^28 elements.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import gpuprefixsums_amd as gps  # noqa: E402
+from tests.gpu_util import (empty_u32, first_mismatch, from_dev_u32, sentinel_u32, sync,  # noqa: E402
+                            to_dev)
+
+ALGOS = {
+    "csdl": dict(algo=gps.ALGO_CSDL),
+    "rts": dict(algo=gps.ALGO_RTS),
+    "csdldf": dict(algo=gps.ALGO_CSDLDF),
+    "csdldf_stall7": dict(algo=gps.ALGO_CSDLDF, stall_period=7),
+    "csdldf_stall8": dict(algo=gps.ALGO_CSDLDF, stall_period=8),
+}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+
+
+def run_scan(h, x: np.ndarray, inclusive: bool) -> np.ndarray:
+    n = x.size
+    d_in = to_dev(x) if n else empty_u32(1)
+    d_out = sentinel_u32(n + 64)  # guard region after n must stay untouched
+    h.scan(d_in, d_out, n, inclusive)
+    sync()
+    out = from_dev_u32(d_out)
+    assert np.all(out[n:n + 64] == 0xDEADBEEF), "wrote past n"
+    return out[:n]
+
+
+def check(h, x, inclusive, label=""):
+    got = run_scan(h, x, inclusive)
+    want = oracle.scan_u32(x, inclusive)
+    assert np.array_equal(got, want), f"{label} n={x.size} incl={inclusive}: {first_mismatch(got, want)}"
+
+
+def sweep_sizes(tile: int):
+    s = {1, 2, 3, 4, 5, 7, 8, 15, 16, 17, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025}
+    for k in (1, 2, 3):
+        s.update(range(k * tile - 5, k * tile + 6))
+    s.update({10 * tile + 3, 64 * tile + 1, 100 * tile - 1, (1 << 20) - 1, 1 << 20, (1 << 20) + 3})
+    return sorted(v for v in s if v > 0)
+
+
+@pytest.mark.parametrize("name", list(ALGOS))
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_size_sweep_full_range(name, inclusive):
+    with gps.ScanHandle(max_n=1 << 21, **ALGOS[name]) as h:
+        for n in sweep_sizes(h.tile):
+            x = oracle.gen_u32(1000 + n, n, 32)
+            check(h, x, inclusive, name)
+        assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_config1_n2p20_values_0_255(inclusive):
+    # n = 2^20, uniform [0, 255], seed 1; CSDL against the host scan.
+    x = oracle.gen_u32(1, 1 << 20, 8)
+    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 20) as h:
+        check(h, x, inclusive, "csdl")
+        assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("name", list(ALGOS))
+def test_reference_d3d12_random_sweep(name):
+    # GPUPrefixSumBase.h:141-148: random [0,1023] (Utility.hlsl InitRandom) with
+    # seed = n for n in [3072, 6144); every 7th size here to bound the time.
+    with gps.ScanHandle(max_n=8192, **ALGOS[name]) as h:
+        for n in range(3072, 6144, 7):
+            x = oracle.gen_d3d12_taus(n, n)
+            for inclusive in (True, False):
+                check(h, x, inclusive, name)
+
+
+@pytest.mark.parametrize("name", list(ALGOS))
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_reference_ones_sweep_one_to_two_tiles(name, inclusive):
+    # ...Dispatcher.cuh:48,88: all-ones input for every n in [T, 2T], checked
+    # with the closed form on the device (UtilityKernels.cuh:22-38).
+    with gps.ScanHandle(max_n=1 << 16, **ALGOS[name]) as h:
+        T = h.tile
+        buf = empty_u32(2 * T)
+        fails = []
+        for n in range(T, 2 * T + 1):
+            gps.fill_u32(buf, n, 1)
+            h.scan(buf, buf, n, inclusive)
+            if gps.validate_ones_u32(buf, n, inclusive):
+                fails.append(n)
+        assert not fails, f"failed sizes {fails[:10]}"
+
+
+@pytest.mark.parametrize("block,vec", [(b, v) for b in (256, 512, 1024) for v in (2, 4, 8)])
+@pytest.mark.parametrize("algo", [gps.ALGO_CSDL, gps.ALGO_RTS, gps.ALGO_CSDLDF])
+def test_every_tile_shape(block, vec, algo):
+    with gps.ScanHandle(algo, 1 << 20, block_threads=block, vec_per_thread=vec,
+                        stall_period=7 if algo == gps.ALGO_CSDLDF else 0) as h:
+        T = h.tile
+        assert T == block * vec * 4
+        for n in (1, T - 1, T, T + 1, 3 * T + 5, (1 << 20) - 3):
+            x = oracle.gen_u32(n * 3 + block + vec, n, 32)
+            check(h, x, True, f"{block}x{vec}")
+            check(h, x, False, f"{block}x{vec}")
+        assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("name", list(ALGOS))
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_in_place(name, inclusive):
+    n = (1 << 20) + 13
+    x = oracle.gen_u32(77, n, 32)
+    with gps.ScanHandle(max_n=n, **ALGOS[name]) as h:
+        d = to_dev(x)
+        h.scan(d, d, n, inclusive)
+        sync()
+        got = from_dev_u32(d)
+        want = oracle.scan_u32(x, inclusive)
+        assert np.array_equal(got, want), first_mismatch(got, want)
+
+
+@pytest.mark.parametrize("name", ["csdl", "rts", "csdldf_stall8"])
+@pytest.mark.parametrize("shift_in,shift_out", [(1, 0), (0, 3), (2, 1), (3, 3)])
+def test_unaligned_buffers(name, shift_in, shift_out):
+    # 4-byte aligned, not 16-byte aligned: the element-wise path.
+    n = 50_001
+    x = oracle.gen_u32(5, n, 32)
+    with gps.ScanHandle(max_n=n, **ALGOS[name]) as h:
+        base_in = to_dev(np.concatenate([np.zeros(shift_in, np.uint32), x]))
+        base_out = sentinel_u32(n + shift_out + 16)
+        d_in = base_in.data_ptr() + 4 * shift_in
+        d_out = base_out.data_ptr() + 4 * shift_out
+        for inclusive in (True, False):
+            h.scan(d_in, d_out, n, inclusive)
+            sync()
+            out = from_dev_u32(base_out)
+            got = out[shift_out:shift_out + n]
+            want = oracle.scan_u32(x, inclusive)
+            assert np.array_equal(got, want), first_mismatch(got, want)
+            assert np.all(out[:shift_out] == 0xDEADBEEF) and np.all(out[shift_out + n:] == 0xDEADBEEF)
+
+
+@pytest.mark.parametrize("name", list(ALGOS))
+def test_all_max_values_wrap(name):
+    n = 300_007
+    x = np.full(n, 0xFFFFFFFF, np.uint32)
+    with gps.ScanHandle(max_n=n, **ALGOS[name]) as h:
+        got = run_scan(h, x, True)
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        assert np.array_equal(got, ((np.uint64(1 << 32) - k) & np.uint64(0xFFFFFFFF)).astype(np.uint32))
+
+
+def test_argument_errors():
+    from gpuprefixsums_amd._lib import GPS_ERR_INVALID, GPS_ERR_TOO_LARGE
+    with gps.ScanHandle(gps.ALGO_CSDL, 1000) as h:
+        d = empty_u32(2000)
+        h.inclusive(d, d, 0)  # n = 0 is a no-op
+        with pytest.raises(gps.GpsError) as e:
+            h.inclusive(d, d, 1001)
+        assert e.value.code == GPS_ERR_TOO_LARGE
+        with pytest.raises(gps.GpsError) as e:
+            h.inclusive(d.data_ptr() + 2, d, 10)
+        assert e.value.code == GPS_ERR_INVALID
+        with pytest.raises(gps.GpsError) as e:
+            h.inclusive(0, d, 10)
+        assert e.value.code == GPS_ERR_INVALID
+
+
+def test_many_launches_and_reset():
+    # Epoch-tagged status words: thousands of back-to-back launches on one
+    # handle with no clearing in between, with an explicit re-arm midway.
+    n = 40_000
+    x = oracle.gen_u32(8, n, 32)
+    want = oracle.scan_u32(x, True)
+    with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+        d_in, d_out = to_dev(x), empty_u32(n)
+        for i in range(3000):
+            h.inclusive(d_in, d_out, n - (i % 5000))
+            if i == 1500:
+                h.reset()
+        h.inclusive(d_in, d_out, n)
+        sync()
+        assert np.array_equal(from_dev_u32(d_out, n), want)
+        assert h.device_error() == 0
+
+
+def test_fill_random_matches_host_generator():
+    n = 100_000
+    d = empty_u32(n)
+    for bits in (8, 10, 32):
+        gps.fill_random_u32(d, n, seed=2, bits=bits)
+        sync()
+        assert np.array_equal(from_dev_u32(d, n), oracle.gen_u32(2, n, bits))
+
+
+def test_device_validators():
+    n = 70_001
+    x = oracle.gen_u32(3, n, 32)
+    d_in = to_dev(x)
+    for inclusive in (True, False):
+        s = oracle.scan_u32(x, inclusive)
+        d_s = to_dev(s)
+        assert gps.validate_scan_u32(d_in, d_s, n, inclusive) == 0
+        s2 = s.copy()
+        s2[n // 2] += 1  # one wrong element breaks two differences
+        assert gps.validate_scan_u32(d_in, to_dev(s2), n, inclusive) == 2
+    ones = to_dev(np.arange(1, n + 1, dtype=np.uint32))
+    assert gps.validate_ones_u32(ones, n, True) == 0
+    assert gps.validate_ones_u32(ones, n, False) == n
+
+
+@pytest.mark.parametrize("period", [7, 8])
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_emulated_deadlock_counters(period, inclusive):
+    n = 1 << 22
+    x = oracle.gen_u32(21, n, 32)
+    with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period, spin_limit=4) as h:
+        h.reset_counters()
+        check(h, x, inclusive, f"stall{period}")
+        c = h.counters()
+        tiles = -(-n // h.tile)
+        stalled = len([t for t in range(0, tiles - 1, period)])
+        # every stalled tile is released only by a successor's insertion
+        assert c["stall_timeouts"] == 0
+        assert c["insertions"] >= stalled
+        assert c["fallbacks"] >= c["insertions"]
+        assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("algo", [gps.ALGO_CSDL, gps.ALGO_RTS, gps.ALGO_CSDLDF])
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_dispatcher_test_all(algo, inclusive):
+    # The reference's TestAll* methods, run by the C++ dispatcher classes.
+    d = {gps.ALGO_CSDL: gps.ChainedScanDecoupledLookbackDispatcher,
+         gps.ALGO_RTS: gps.ReduceThenScanDispatcher,
+         gps.ALGO_CSDLDF: gps.EmulatedDeadlockingDispatcher}[algo](1 << 22)
+    passed, total = d.test_all_inclusive() if inclusive else d.test_all_exclusive()
+    assert total > 4000 and passed == total
