@@ -17,7 +17,8 @@ import ctypes
 from typing import Optional, Sequence
 
 from . import _lib
-from ._lib import ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, GpsError, GpsTuning, check, load
+from ._lib import (ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, SCHEDULE_DEFAULT, SCHEDULE_PERSISTENT, SCHEDULE_TILE,
+                   GpsError, GpsTuning, check, load)
 
 __all__ = [
     "ALGO_CSDL", "ALGO_RTS", "ALGO_CSDLDF", "GpsError", "ScanHandle",
@@ -58,9 +59,9 @@ class ScanHandle:
     scans of up to max_n elements. Not re-entrant: use one per stream."""
 
     def __init__(self, algo: int = ALGO_CSDL, max_n: int = 1 << 28, *, block_threads: int = 0,
-                 vec_per_thread: int = 0, spin_limit: int = 0, stall_period: int = 0):
+                 vec_per_thread: int = 0, spin_limit: int = 0, stall_period: int = 0, schedule: int = 0):
         L = load()
-        t = GpsTuning(block_threads, vec_per_thread, spin_limit, stall_period)
+        t = GpsTuning(block_threads, vec_per_thread, spin_limit, stall_period, schedule)
         h = ctypes.c_void_p()
         check(L.gps_create(algo, max_n, ctypes.byref(t), ctypes.byref(h)))
         self._h = h
@@ -73,6 +74,7 @@ class ScanHandle:
         self.vec_per_thread = out.vec_per_thread
         self.spin_limit = out.spin_limit
         self.stall_period = out.stall_period
+        self.schedule = out.schedule
         self.tile = tile.value
 
     @property

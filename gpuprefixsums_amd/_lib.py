@@ -34,7 +34,13 @@ class GpsTuning(ctypes.Structure):
         ("vec_per_thread", _u32),
         ("spin_limit", _u32),
         ("stall_period", _u32),
+        ("schedule", _u32),
     ]
+
+
+SCHEDULE_DEFAULT = 0
+SCHEDULE_TILE = 1
+SCHEDULE_PERSISTENT = 2
 
 
 class GpsError(RuntimeError):

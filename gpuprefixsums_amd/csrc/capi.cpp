@@ -45,6 +45,7 @@ struct gps_handle {
     gps::TileShape shape{256, 4};
     uint32_t spin_limit = 4;
     uint32_t stall_period = 0;
+    uint32_t schedule = GPS_SCHEDULE_TILE;
     uint32_t max_tiles = 0;
     int device = 0;
     uint64_t* status = nullptr;
@@ -57,16 +58,21 @@ struct gps_handle {
 
 namespace {
 
+// Defaults from tile-shape sweeps at n = 2^28 on MI355X. Single-pass scans
+// want large tiles: each tile costs one device-scope ticket atomic and one
+// status round trip, both serialised at the coherence point.
 gps::TileShape default_shape(int algo) {
     switch (algo) {
         case GPS_ALGO_RTS:
-            return {256, 8};
+            return {512, 2};
         case GPS_ALGO_CSDLDF:
-            return {256, 4};
+            return {1024, 4};
         default:
-            return {256, 4};
+            return {1024, 8};
     }
 }
+
+uint32_t default_schedule(int algo) { return algo == GPS_ALGO_CSDL ? GPS_SCHEDULE_PERSISTENT : GPS_SCHEDULE_TILE; }
 
 void free_handle(gps_handle* h) {
     if (!h) return;
@@ -125,6 +131,8 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
     p.counters = h->counters;
     if (h->algo == GPS_ALGO_CSDLDF) {
         HIP_TRY(gps::launch_csdldf_u32(h->shape, inclusive, p, s));
+    } else if (h->schedule == GPS_SCHEDULE_PERSISTENT) {
+        HIP_TRY(gps::launch_csdl_pp_u32(h->shape, inclusive, p, s));
     } else {
         HIP_TRY(gps::launch_csdl_u32(h->shape, inclusive, p, s));
     }
@@ -147,13 +155,18 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
         return fail(GPS_ERR_INVALID, "unknown algorithm %d", algo);
     if (max_n == 0) return fail(GPS_ERR_INVALID, "max_n must be positive");
     gps::TileShape shape = default_shape(algo);
-    uint32_t spin_limit = 4, stall_period = 0;
+    uint32_t spin_limit = 4, stall_period = 0, schedule = GPS_SCHEDULE_DEFAULT;
     if (tuning) {
         if (tuning->block_threads) shape.block = tuning->block_threads;
         if (tuning->vec_per_thread) shape.vec = tuning->vec_per_thread;
         if (tuning->spin_limit) spin_limit = tuning->spin_limit;
         stall_period = tuning->stall_period;
+        schedule = tuning->schedule;
     }
+    if (schedule > GPS_SCHEDULE_PERSISTENT) return fail(GPS_ERR_INVALID, "unknown schedule %u", schedule);
+    if (schedule == GPS_SCHEDULE_PERSISTENT && algo != GPS_ALGO_CSDL)
+        return fail(GPS_ERR_INVALID, "the persistent schedule is offered for CSDL only");
+    if (schedule == GPS_SCHEDULE_DEFAULT) schedule = default_schedule(algo);
     if (!gps::shape_supported(shape))
         return fail(GPS_ERR_INVALID, "unsupported tile shape block=%u vec=%u (block 256/512/1024, vec 2/4/8)",
                     shape.block, shape.vec);
@@ -169,6 +182,7 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     h->shape = shape;
     h->spin_limit = spin_limit;
     h->stall_period = stall_period;
+    h->schedule = schedule;
     h->max_tiles = (uint32_t)tiles;
     hipError_t e = hipGetDevice(&h->device);
     if (e == hipSuccess) e = hipMalloc(&h->status, tiles * sizeof(uint64_t));
@@ -200,6 +214,7 @@ int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems) {
     out->vec_per_thread = h->shape.vec;
     out->spin_limit = h->spin_limit;
     out->stall_period = h->stall_period;
+    out->schedule = h->schedule;
     if (tile_elems) *tile_elems = h->shape.tile();
     return GPS_OK;
 }

@@ -110,14 +110,17 @@ struct Ticket {
     uint32_t epoch;  // already masked to kEpochBits
 };
 
-// Called by one thread.
-__device__ __forceinline__ Ticket take_ticket(uint64_t* ticket_word, uint32_t num_tiles) {
+// Called by one thread. `total_tickets` is the number of tickets the launch
+// takes in all: the tile count for one-tile-per-workgroup kernels, tiles +
+// workgroups for persistent kernels (each persistent workgroup ends on one
+// ticket past the last tile).
+__device__ __forceinline__ Ticket take_ticket(uint64_t* ticket_word, uint32_t total_tickets) {
     const uint64_t old = __hip_atomic_fetch_add(ticket_word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     Ticket t;
     t.tile = (uint32_t)old;
     const uint32_t e = (uint32_t)(old >> 32);
     t.epoch = e & kEpochMask;
-    if (t.tile == num_tiles - 1u) atomic_store_agent(ticket_word, (uint64_t)next_epoch(e) << 32);
+    if (t.tile == total_tickets - 1u) atomic_store_agent(ticket_word, (uint64_t)next_epoch(e) << 32);
     return t;
 }
 
@@ -140,25 +143,43 @@ constexpr uint32_t kErrLookbackWatchdog = 1u;
 constexpr uint32_t kErrStallWatchdog = 2u;
 
 // ---------------------------------------------------------------------------
-// Window lookback (one full wave): returns the exclusive prefix of `tile`,
-// i.e. the sum mod 2^32 of the aggregates of all tiles before it.
+// Window lookback (one full wave) for the exclusive prefix of a tile, i.e. the
+// sum mod 2^32 of the aggregates of all tiles before it.
 // Lane l reads the word of tile (top - l), top starting at tile - 1. The wave
 // consumes the contiguous ready run nearest to the tile: if that run contains
 // an inclusive word, the prefix is complete; otherwise the run's aggregates
 // are added and the window slides below it. A not-ready nearest word is
-// polled again (s_sleep between polls). Unlike the reference's warp lookback,
-// which waits until all 32 words of its window are ready, the wave never waits
-// on a word it does not need.
+// polled again. Unlike the reference's warp lookback, which waits until all
+// 32 words of its window are ready, the wave never waits on a word it does
+// not need. (Wider windows, several words per lane per poll, measured slower
+// on MI355X: the polls' latency, not their count, dominates.)
+//
+// Split into issue() and consume() so a caller can put independent memory
+// traffic (the next tile's loads) between a poll and its use.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t lookback_u32(const uint64_t* status, uint32_t tile, uint32_t epoch,
-                                                 uint32_t* err) {
-    const uint32_t lane = lane_id();
-    uint32_t prefix = 0;
-    int64_t top = (int64_t)tile - 1;
-    uint32_t empty_polls = 0;
-    while (true) {
-        const int64_t idx = top - (int64_t)lane;
-        const uint64_t w = idx >= 0 ? atomic_load_agent(status + idx) : make_status(0u, kFlagIncl, epoch);
+#ifndef GPS_POLL_SLEEP
+#define GPS_POLL_SLEEP 1  // s_sleep argument between empty polls (x 64 clocks)
+#endif
+
+struct Lookback {
+    uint32_t prefix;
+    int64_t top;
+    uint64_t w;  // this lane's word of the last issued window
+    uint32_t empty_polls;
+
+    __device__ __forceinline__ void start(uint32_t tile) {
+        prefix = 0;
+        top = (int64_t)tile - 1;
+        empty_polls = 0;
+    }
+    __device__ __forceinline__ void issue(const uint64_t* status, uint32_t epoch) {
+        const int64_t idx = top - (int64_t)lane_id();
+        w = idx >= 0 ? atomic_load_agent(status + idx) : make_status(0u, kFlagIncl, epoch);
+    }
+    // Folds the issued window in; true once `prefix` is complete. Sets
+    // `gave_up` (and returns true) when the watchdog fires.
+    __device__ __forceinline__ bool consume(uint32_t epoch, uint32_t* err, bool& gave_up) {
+        const uint32_t lane = lane_id();
         const uint32_t f = status_flag(w, epoch);
         const uint64_t ready = ballot(f != 0u);
         const uint64_t incl = ballot(f == (uint32_t)kFlagIncl);
@@ -166,7 +187,7 @@ __device__ __forceinline__ uint32_t lookback_u32(const uint64_t* status, uint32_
         const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
         if (first_incl < run) {
             prefix += wave_sum(lane <= first_incl ? (uint32_t)w : 0u);
-            return prefix;
+            return true;
         }
         if (run) {
             prefix += wave_sum(lane < run ? (uint32_t)w : 0u);
@@ -175,11 +196,28 @@ __device__ __forceinline__ uint32_t lookback_u32(const uint64_t* status, uint32_
         } else {
             if (watchdog_expired(empty_polls, err)) {
                 if (lane == 0) raise_device_error(err, kErrLookbackWatchdog);
-                return prefix;
+                gave_up = true;
+                return true;
             }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(GPS_POLL_SLEEP);
         }
+        return false;
     }
+    // Polls until complete (after an optional already-issued window).
+    __device__ __forceinline__ uint32_t finish(const uint64_t* status, uint32_t epoch, uint32_t* err,
+                                               bool issued) {
+        bool gave_up = false;
+        if (!issued) issue(status, epoch);
+        while (!consume(epoch, err, gave_up)) issue(status, epoch);
+        return prefix;
+    }
+};
+
+__device__ __forceinline__ uint32_t lookback_u32(const uint64_t* status, uint32_t tile, uint32_t epoch,
+                                                 uint32_t* err) {
+    Lookback lb;
+    lb.start(tile);
+    return lb.finish(status, epoch, err, false);
 }
 
 }  // namespace gps

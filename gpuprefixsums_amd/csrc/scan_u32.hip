@@ -63,8 +63,17 @@ struct TileFront {
 // CSDL. Wave 0 publishes the tile aggregate, looks back, and publishes the
 // inclusive prefix; every thread then adds tile prefix + wave prefix.
 // ---------------------------------------------------------------------------
+// Occupancy target for the single-pass kernels: 8 waves per SIMD (<= 64
+// VGPRs), i.e. 2 x 1024, 4 x 512 or 8 x 256 threads per CU. A workgroup holds
+// its loaded tile in registers through the lookback, so resident bytes in
+// flight, not ALU, set the achieved bandwidth.
+#ifndef GPS_WAVES_PER_EU
+#define GPS_WAVES_PER_EU 8
+#endif
+
 template <int BLOCK, int VEC, bool INCL>
-__global__ __launch_bounds__(BLOCK) void csdl_u32(ScanParams p) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES_PER_EU))) void csdl_u32(
+    ScanParams p) {
     using C = TileCfg<BLOCK, VEC>;
     __shared__ uint32_t s_wave[C::kWaves];
     __shared__ uint32_t s_ticket[2];
@@ -91,6 +100,106 @@ __global__ __launch_bounds__(BLOCK) void csdl_u32(ScanParams p) {
     }
     __syncthreads();
     f.store(p, s_prefix + f.wave_prefix);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent, software-pipelined CSDL (the MI355X default schedule).
+//
+// A chained scan's workgroup must wait for its tile's exclusive prefix before
+// it can store, and on MI355X one status round trip to the cross-XCD
+// coherence point under full HBM load costs about as much as a 16K-element
+// tile's loads. Instead of idling through that wait, each workgroup here
+// owns a stream of tickets: with tile i scanned and its aggregate published,
+// it takes ticket i+1 and issues tile i+1's loads, then resolves tile i's
+// lookback (the first poll is issued ahead of the prefetch so the in-order
+// vmcnt does not make it wait for those loads) and stores tile i. The grid is
+// sized to what is resident; every workgroup ends on exactly one ticket past
+// the last tile, so the ticket word re-arms after num_tiles + gridDim.x
+// tickets. Two register tiles per thread (current + prefetched).
+//
+// Forward progress: a tile is only ever held by a running workgroup, and the
+// lowest tile without a published aggregate belongs to a workgroup whose
+// current tile is lower still and already has every predecessor published.
+// ---------------------------------------------------------------------------
+template <int BLOCK, int VEC, bool INCL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(VEC >= 8 ? 4 : GPS_WAVES_PER_EU))) void
+csdl_pp_u32(ScanParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ uint32_t s_wave[C::kWaves];
+    __shared__ uint32_t s_ticket[3];
+    __shared__ uint32_t s_prefix;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, total_tickets);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    __syncthreads();
+    uint32_t tile = s_ticket[0];
+    const uint32_t epoch = s_ticket[1];
+    if (tile >= p.num_tiles) return;
+
+    u32x4 a[VEC], b[VEC];
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
+        load_vec<VEC>(a, p.in + base);
+    else
+        load_guarded<VEC>(a, p.in, base, p.n);
+
+    while (true) {
+        const uint32_t run = wave_scan_run<VEC, INCL>(a);
+        if (lane == 0) s_wave[wave] = run;
+        __syncthreads();
+        uint32_t wave_prefix, tile_total;
+        combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
+        if (threadIdx.x == 0) {
+            // Publish first, then take the next ticket: ticket order then
+            // follows publication order, so lookbacks find contiguous ready
+            // runs. (Taking the ticket before the scan, or a whole tile
+            // ahead, measured 8% slower at 2^28.)
+            atomic_store_agent(p.status + tile, make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+        }
+        __syncthreads();
+        const uint32_t next = s_ticket[2];
+        const bool has_next = next < p.num_tiles;
+
+        Lookback lb;
+        if (wave == 0 && tile != 0) {
+            lb.start(tile);
+            lb.issue(p.status, epoch);
+        }
+        if (has_next) {
+            const uint64_t next_base = thread_base<BLOCK, VEC>(next);
+            if (p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n))
+                load_vec<VEC>(b, p.in + next_base);
+            else
+                load_guarded<VEC>(b, p.in, next_base, p.n);
+        }
+        if (wave == 0) {
+            uint32_t prefix = 0;
+            if (tile != 0) {
+                prefix = lb.finish(p.status, epoch, p.err, true);
+                if (lane == 0)
+                    atomic_store_agent(p.status + tile, make_status(prefix + tile_total, kFlagIncl, epoch));
+            }
+            if (lane == 0) s_prefix = prefix;
+        }
+        __syncthreads();
+        const uint32_t add = s_prefix + wave_prefix;
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
+            store_vec<VEC>(p.out + base, a, add);
+        else
+            store_guarded<VEC>(p.out, a, add, base, p.n);
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) a[k] = b[k];
+        tile = next;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -454,6 +563,38 @@ hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hip
         else                                                                             \
             hipLaunchKernelGGL((csdl_u32<B, V, false>), dim3(p.num_tiles), dim3(B), 0, stream, p); \
         return hipGetLastError();                                                        \
+    }
+    GPS_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// Persistent grid: as many workgroups as fit on the device at once (the
+// occupancy query is made once per kernel instance and process).
+template <int B, int V, bool I>
+static hipError_t launch_pp(const ScanParams& p, hipStream_t stream) {
+    struct Residency {
+        uint32_t workgroups = 0;
+        Residency() {
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess) return;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, csdl_pp_u32<B, V, I>, B, 0) != hipSuccess)
+                return;
+            workgroups = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+        }
+    };
+    static const Residency r;
+    if (r.workgroups == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = p.num_tiles < r.workgroups ? p.num_tiles : r.workgroups;
+    hipLaunchKernelGGL((csdl_pp_u32<B, V, I>), dim3(grid), dim3(B), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+#define X(B, V)                                                                          \
+    if (s.block == B && s.vec == V) {                                                    \
+        return inclusive ? launch_pp<B, V, true>(p, stream) : launch_pp<B, V, false>(p, stream); \
     }
     GPS_SHAPES(X)
 #undef X

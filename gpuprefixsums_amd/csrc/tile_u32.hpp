@@ -117,18 +117,16 @@ __device__ __forceinline__ bool tile_is_full(uint32_t tile, uint64_t n) {
 
 // Exclusive prefix of this wave inside the tile, and the tile total, from the
 // per-wave totals in LDS (after a barrier).
+// Lane j loads wave j's total and the wave scans them with DPP, so each
+// thread needs one LDS read and two registers whatever the block size.
 template <int WAVES>
 __device__ __forceinline__ void combine_waves(const uint32_t* s_wave, uint32_t wave, uint32_t& wave_prefix,
                                               uint32_t& tile_total) {
-    uint32_t p = 0, t = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < (uint32_t)WAVES; ++j) {
-        const uint32_t v = s_wave[j];
-        p += j < wave ? v : 0u;
-        t += v;
-    }
-    wave_prefix = p;
-    tile_total = t;
+    const uint32_t lane = lane_id();
+    const uint32_t v = lane < (uint32_t)WAVES ? s_wave[lane] : 0u;
+    const uint32_t s = wave_incl_scan(v);
+    wave_prefix = __builtin_amdgcn_readlane(s - v, wave);
+    tile_total = readlane(s, WAVES - 1);
 }
 
 }  // namespace gps

@@ -53,7 +53,16 @@ typedef struct gps_tuning {
     uint32_t vec_per_thread; /* uint4 vectors per thread: 2, 4 or 8 (items = 4x) */
     uint32_t spin_limit;     /* CSDLDF: polls without progress before a fallback (default 4) */
     uint32_t stall_period;   /* CSDLDF: stall tiles with index % period == 0 (0 = no emulation, >= 2) */
+    uint32_t schedule;       /* CSDL: GPS_SCHEDULE_* (0 = default) */
 } gps_tuning;
+
+/* CSDL workgroup schedules. TILE: one workgroup per tile, as the reference
+ * launches (ChainedScanDecoupledLookbackDispatcher.cuh:192-194). PERSISTENT:
+ * resident workgroups loop over ticketed tiles and load tile i+1 while tile
+ * i's lookback resolves. */
+#define GPS_SCHEDULE_DEFAULT 0
+#define GPS_SCHEDULE_TILE 1
+#define GPS_SCHEDULE_PERSISTENT 2
 
 typedef struct gps_handle gps_handle;
 

@@ -21,7 +21,8 @@ from tests.gpu_util import (empty_u32, first_mismatch, from_dev_u32, sentinel_u3
                             to_dev)
 
 ALGOS = {
-    "csdl": dict(algo=gps.ALGO_CSDL),
+    "csdl": dict(algo=gps.ALGO_CSDL),  # persistent pipelined schedule (default)
+    "csdl_tile": dict(algo=gps.ALGO_CSDL, schedule=gps.SCHEDULE_TILE),  # one workgroup per tile
     "rts": dict(algo=gps.ALGO_RTS),
     "csdldf": dict(algo=gps.ALGO_CSDLDF),
     "csdldf_stall7": dict(algo=gps.ALGO_CSDLDF, stall_period=7),
@@ -63,7 +64,7 @@ def sweep_sizes(tile: int):
 @pytest.mark.parametrize("name", list(ALGOS))
 @pytest.mark.parametrize("inclusive", [True, False])
 def test_size_sweep_full_range(name, inclusive):
-    with gps.ScanHandle(max_n=1 << 21, **ALGOS[name]) as h:
+    with gps.ScanHandle(max_n=1 << 23, **ALGOS[name]) as h:
         for n in sweep_sizes(h.tile):
             x = oracle.gen_u32(1000 + n, n, 32)
             check(h, x, inclusive, name)
@@ -108,10 +109,9 @@ def test_reference_ones_sweep_one_to_two_tiles(name, inclusive):
 
 
 @pytest.mark.parametrize("block,vec", [(b, v) for b in (256, 512, 1024) for v in (2, 4, 8)])
-@pytest.mark.parametrize("algo", [gps.ALGO_CSDL, gps.ALGO_RTS, gps.ALGO_CSDLDF])
-def test_every_tile_shape(block, vec, algo):
-    with gps.ScanHandle(algo, 1 << 20, block_threads=block, vec_per_thread=vec,
-                        stall_period=7 if algo == gps.ALGO_CSDLDF else 0) as h:
+@pytest.mark.parametrize("name", ["csdl", "csdl_tile", "rts", "csdldf_stall7"])
+def test_every_tile_shape(block, vec, name):
+    with gps.ScanHandle(max_n=1 << 20, block_threads=block, vec_per_thread=vec, **ALGOS[name]) as h:
         T = h.tile
         assert T == block * vec * 4
         for n in (1, T - 1, T, T + 1, 3 * T + 5, (1 << 20) - 3):
