@@ -3,12 +3,15 @@
 n = 2^28 full-range values on one MI355X, as GB/s at 8 algorithmic bytes per
 element (4 read + 4 write) and as a fraction of the 8 TB/s HBM peak.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs all|none|a,b,..] [--no-cpu-baseline]
 
 One step = one scan of the whole 2^28-element input (1 GiB in, 1 GiB out,
 separate buffers, input resident in HBM). Prints ONE JSON line on stdout;
 progress goes to stderr. The CPU baseline is the serial host scan (the
 reference's validator loop restated in C) timed on this machine's host.
+
+Extra legs (reported under "extras"): csdl_exclusive, rts, csdldf (stall
+periods 0/7/8), copy (the read-n + write-n ceiling), n2p20 (latency at 2^20).
 """
 from __future__ import annotations
 
@@ -26,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "n2p20"]
 
 
 def log(msg: str) -> None:
@@ -47,6 +51,7 @@ def gbps(n: int, ms: float, bytes_per_elem: float = BYTES_PER_ELEM) -> float:
 
 
 def load_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary)."""
     p = ROOT / "profiles" / f"{name}_traffic.json"
     if not p.exists():
         return None
@@ -56,19 +61,89 @@ def load_traffic(name: str):
         return None
 
 
+class Legs:
+    """The extra measurements, each on the same resident 2^28 input."""
+
+    def __init__(self, gps, torch, oracle, np, d_in, d_out, n):
+        self.gps, self.torch, self.oracle, self.np = gps, torch, oracle, np
+        self.d_in, self.d_out, self.n = d_in, d_out, n
+
+    def _timed(self, handle, inclusive=True, iters=20):
+        gps, n = self.gps, self.n
+        for _ in range(3):
+            handle.scan(self.d_in, self.d_out, n, inclusive)
+        t = handle.time(self.d_in, self.d_out, n, inclusive, iters=iters)
+        if gps.validate_scan_u32(self.d_in, self.d_out, n, inclusive) or handle.device_error():
+            raise RuntimeError("validation failed")
+        return statistics.median(t)
+
+    def csdl_exclusive(self, out):
+        with self.gps.ScanHandle(self.gps.ALGO_CSDL, self.n) as h:
+            out["csdl_exclusive_gbps"] = round(gbps(self.n, self._timed(h, inclusive=False)), 1)
+
+    def rts(self, out):
+        with self.gps.ScanHandle(self.gps.ALGO_RTS, self.n) as h:
+            m = self._timed(h)
+            out["rts_inclusive_gbps"] = round(gbps(self.n, m), 1)
+            out["rts_physical_gbps_12B"] = round(gbps(self.n, m, 12), 1)
+
+    def csdldf(self, out):
+        gps = self.gps
+        for period in (0, 7, 8):
+            with gps.ScanHandle(gps.ALGO_CSDLDF, self.n, stall_period=period) as h:
+                for _ in range(3):
+                    h.inclusive(self.d_in, self.d_out, self.n)
+                h.reset_counters()
+                m = self._timed(h, iters=20)
+                c = h.counters()
+                launches = 23  # 3 warm-up + 20 timed since the reset
+                out[f"csdldf_stall{period}_gbps"] = round(gbps(self.n, m), 1)
+                out[f"csdldf_stall{period}_per_launch"] = {k: round(v / launches, 1) for k, v in c.items()}
+                out[f"csdldf_stall{period}_tiles"] = -(-self.n // h.tile)
+
+    def copy(self, out):
+        gps, torch = self.gps, self.torch
+        st = torch.cuda.current_stream()
+        for _ in range(3):
+            gps.copy_u32(self.d_out, self.d_in, self.n)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(20):
+            gps.copy_u32(self.d_out, self.d_in, self.n)
+        e1.record(st)
+        torch.cuda.synchronize()
+        out["copy_ceiling_gbps"] = round(gbps(self.n, e0.elapsed_time(e1) / 20), 1)
+
+    def n2p20(self, out):
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        n1 = 1 << 20
+        x1 = torch.from_numpy(oracle.gen_u32(1, n1, 8).view(np.int32)).to("cuda")
+        o1 = torch.empty_like(x1)
+        with gps.ScanHandle(gps.ALGO_CSDL, n1) as h1:
+            for _ in range(5):
+                h1.inclusive(x1, o1, n1)
+            t = h1.time(x1, o1, n1, True, iters=50)
+            out["n2p20_csdl_inclusive_us"] = round(statistics.median(t) * 1e3, 2)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--log2n", type=int, default=28)
-    ap.add_argument("--extras", action=argparse.BooleanOptionalAction, default=True)
+    ap.add_argument("--legs", default="all", help="all, none, or a comma list of " + ",".join(ALL_LEGS))
     ap.add_argument("--cpu-baseline", action=argparse.BooleanOptionalAction, default=True)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this library is single-GPU (the multi-GPU "
               "sharding of the scan is a design note, not built); run with --gpus 1.", file=sys.stderr)
+        return 2
+    legs = ALL_LEGS if args.legs == "all" else [] if args.legs == "none" else args.legs.split(",")
+    unknown = [x for x in legs if x not in ALL_LEGS]
+    if unknown:
+        print(f"unknown legs {unknown}", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -84,7 +159,7 @@ def main() -> int:
     K, W = args.steps, args.warmup
     seed = 2
 
-    log(f"generating n=2^{args.log2n} full-range uint32 input (seed {seed}) on the host")
+    log(f"generating n=2^{args.log2n} full-range uint32 input (splitmix64 seed {seed}) on the host")
     x = oracle.gen_u32(seed, n, 32)
     d_in = torch.from_numpy(x.view(np.int32)).to("cuda")
     d_out = torch.empty(n, dtype=torch.int32, device="cuda")
@@ -95,8 +170,9 @@ def main() -> int:
         h.inclusive(d_in, d_out, n)
     torch.cuda.synchronize()
 
-    # Timed region: K scans, each bracketed by hipEvents on the launch stream
-    # (the null stream, torch's current stream); synchronize on both sides.
+    # Timed region: K scans on the null stream (torch's current stream), each
+    # bracketed by hipEvents that the library records on that same stream;
+    # synchronize on both sides for the wall-clock figure.
     t0 = time.perf_counter()
     ms = h.time(d_in, d_out, n, True, iters=K)
     torch.cuda.synchronize()
@@ -113,6 +189,7 @@ def main() -> int:
         f"median {med_ms * 1e3:.1f} us -> {gbps(n, mean_ms):.1f} GB/s")
 
     traffic = load_traffic(f"csdl_u32_n2p{args.log2n}")
+    schedule = {1: "tile", 2: "persistent"}.get(h.schedule, str(h.schedule))
     roofline = {
         "bound": "hbm",
         "achieved": round(gbps(n, mean_ms), 1),
@@ -120,7 +197,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(gbps(n, mean_ms) / HBM_PEAK_GBPS, 4),
         "traffic": traffic.get("bytes_per_launch") if traffic else None,
-        "kernel": f"csdl_u32<{h.block_threads},{h.vec_per_thread},incl>",
+        "kernel": f"csdl_pp_u32<{h.block_threads},{h.vec_per_thread},incl> ({schedule} schedule)",
         "algorithmic_bytes": n * BYTES_PER_ELEM,
         "kernel_ms_mean": round(mean_ms, 5),
         "kernel_ms_median": round(med_ms, 5),
@@ -128,51 +205,13 @@ def main() -> int:
     }
 
     extras = {}
-    if args.extras:
-        def timed(handle, inclusive=True, iters=20):
-            for _ in range(3):
-                handle.scan(d_in, d_out, n, inclusive)
-            t = handle.time(d_in, d_out, n, inclusive, iters=iters)
-            e = gps.validate_scan_u32(d_in, d_out, n, inclusive)
-            if e or handle.device_error():
-                raise RuntimeError("validation failed in extras")
-            return statistics.median(t)
-
-        m = timed(h, inclusive=False)
-        extras["csdl_exclusive_gbps"] = round(gbps(n, m), 1)
-        with gps.ScanHandle(gps.ALGO_RTS, n) as r:
-            m = timed(r)
-            extras["rts_inclusive_gbps"] = round(gbps(n, m), 1)
-        for period in (0, 7, 8):
-            with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period) as f:
-                m = timed(f)
-                c = f.counters()
-                extras[f"csdldf_stall{period}_gbps"] = round(gbps(n, m), 1)
-                extras[f"csdldf_stall{period}_counters_per_launch"] = {
-                    k: round(v / 23, 1) for k, v in c.items()}
-        # copy ceiling: read n + write n with the same access pattern
-        st = torch.cuda.current_stream()
-        for _ in range(3):
-            gps.copy_u32(d_out, d_in, n)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        for _ in range(20):
-            gps.copy_u32(d_out, d_in, n)
-        e1.record(st)
-        torch.cuda.synchronize()
-        copy_ms = e0.elapsed_time(e1) / 20
-        extras["copy_ceiling_gbps"] = round(gbps(n, copy_ms), 1)
-        extras["csdl_frac_of_copy_ceiling"] = round(gbps(n, mean_ms) / gbps(n, copy_ms), 4)
-        # n = 2^20 latency (values [0,255], seed 1)
-        n1 = 1 << 20
-        x1 = torch.from_numpy(oracle.gen_u32(1, n1, 8).view(np.int32)).to("cuda")
-        o1 = torch.empty_like(x1)
-        with gps.ScanHandle(gps.ALGO_CSDL, n1) as h1:
-            for _ in range(5):
-                h1.inclusive(x1, o1, n1)
-            t = h1.time(x1, o1, n1, True, iters=50)
-            extras["n2p20_csdl_inclusive_us"] = round(statistics.median(t) * 1e3, 2)
-        log(f"extras: {json.dumps(extras)}")
+    runner = Legs(gps, torch, oracle, np, d_in, d_out, n)
+    for leg in legs:
+        t = time.perf_counter()
+        getattr(runner, leg)(extras)
+        log(f"leg {leg} done in {time.perf_counter() - t:.1f} s")
+    if "copy_ceiling_gbps" in extras:
+        extras["csdl_frac_of_copy_ceiling"] = round(gbps(n, mean_ms) / extras["copy_ceiling_gbps"], 4)
 
     cpu = None
     if args.cpu_baseline:
@@ -213,7 +252,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"uint32 inclusive scan, CSDL, n=2^{args.log2n}, values uniform in "
                                f"[0, 2^32-1] (splitmix64 seed {seed}), separate 1 GiB in/out",
-                   "n": n, "algo": "csdl", "tile": h.tile, "bytes_per_elem": BYTES_PER_ELEM},
+                   "n": n, "algo": "csdl", "tile": h.tile, "schedule": schedule,
+                   "bytes_per_elem": BYTES_PER_ELEM},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extras": extras,
