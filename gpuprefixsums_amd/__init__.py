@@ -117,6 +117,27 @@ class ScanHandle:
                                        _stream(stream), iters, ms))
         return list(ms)
 
+    def segmented(self, d_in, d_out, d_offsets, num_segments: int, n: int, inclusive: bool = False,
+                  stream=None) -> None:
+        """Batched segmented scan in one launch (d_offsets: device uint64[num_segments + 1])."""
+        check(load().gps_segmented_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), _ptr(d_offsets), num_segments,
+                                            n, int(inclusive), _stream(stream)))
+
+    def segmented_per_segment(self, d_in, d_out, h_offsets, inclusive: bool = False, stream=None) -> None:
+        """Baseline: one plain scan launch per segment (h_offsets: host uint64 sequence)."""
+        import numpy as np
+        off = np.ascontiguousarray(h_offsets, dtype=np.uint64)
+        ptr = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        check(load().gps_segmented_scan_u32_per_segment(self.handle, _ptr(d_in), _ptr(d_out), ptr, off.size - 1,
+                                                        int(inclusive), _stream(stream)))
+
+    def segmented_graph(self, d_in, d_out, d_offsets, num_segments: int, n: int,
+                        inclusive: bool = False) -> "Graph":
+        g = ctypes.c_void_p()
+        check(load().gps_segmented_graph_create(self.handle, _ptr(d_in), _ptr(d_out), _ptr(d_offsets),
+                                                num_segments, n, int(inclusive), ctypes.byref(g)))
+        return Graph(g)
+
     def counters(self) -> dict:
         c = (ctypes.c_uint64 * 4)()
         check(load().gps_get_counters(self.handle, c))
@@ -132,6 +153,33 @@ class ScanHandle:
 
     def reset(self, stream=None) -> None:
         check(load().gps_reset(self.handle, _stream(stream)))
+
+
+class Graph:
+    """An instantiated hipGraph (from ScanHandle.segmented_graph)."""
+
+    def __init__(self, g: ctypes.c_void_p):
+        self._g = g
+
+    def launch(self, stream=None) -> None:
+        check(load().gps_graph_launch(self._g, _stream(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_g", None) is not None:
+            check(load().gps_graph_destroy(self._g))
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # ---- utility kernels -----------------------------------------------------

@@ -59,6 +59,11 @@ SIGNATURES = {
     "gps_get_tuning": (_int, [_vp, ctypes.POINTER(GpsTuning), ctypes.POINTER(_u32)]),
     "gps_inclusive_scan_u32": (_int, [_vp, _vp, _vp, _u64, _vp]),
     "gps_exclusive_scan_u32": (_int, [_vp, _vp, _vp, _u64, _vp]),
+    "gps_segmented_scan_u32": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, _vp]),
+    "gps_segmented_scan_u32_per_segment": (_int, [_vp, _vp, _vp, ctypes.POINTER(_u64), _u64, _int, _vp]),
+    "gps_segmented_graph_create": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, ctypes.POINTER(_vp)]),
+    "gps_graph_launch": (_int, [_vp, _vp]),
+    "gps_graph_destroy": (_int, [_vp]),
     "gps_get_counters": (_int, [_vp, ctypes.POINTER(_u64)]),
     "gps_reset_counters": (_int, [_vp]),
     "gps_device_error": (_int, [_vp, ctypes.POINTER(_u32)]),

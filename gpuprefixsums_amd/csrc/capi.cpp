@@ -7,12 +7,18 @@
 #include <string>
 #include <vector>
 
-#include "../../include/gps.h"
-#include "gps_internal.hpp"
+#include "handle.hpp"
 
 namespace {
 
 thread_local std::string g_last_error;
+
+// Epochs are 30 bits wide; re-arm the status words before a value could recur.
+constexpr uint64_t kLaunchesBeforeReset = (1ull << 30) - 2;
+
+}  // namespace
+
+namespace gps {
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -28,33 +34,20 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(GPS_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
 }
 
-#define HIP_TRY(expr)                                     \
-    do {                                                  \
-        hipError_t e_ = (expr);                           \
-        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
-    } while (0)
+hipError_t before_epoch_launch(gps_handle* h, hipStream_t s) {
+    if (h->launches >= kLaunchesBeforeReset) {
+        const hipError_t e = launch_reset_status(h->status, h->max_tiles, h->ticket, s);
+        if (e != hipSuccess) return e;
+        h->launches = 0;
+    }
+    ++h->launches;
+    return hipSuccess;
+}
 
-// Epochs are 30 bits wide; re-arm the status words before a value could recur.
-constexpr uint64_t kLaunchesBeforeReset = (1ull << 30) - 2;
+}  // namespace gps
 
-}  // namespace
-
-struct gps_handle {
-    int algo = 0;
-    uint64_t max_n = 0;
-    gps::TileShape shape{256, 4};
-    uint32_t spin_limit = 4;
-    uint32_t stall_period = 0;
-    uint32_t schedule = GPS_SCHEDULE_TILE;
-    uint32_t max_tiles = 0;
-    int device = 0;
-    uint64_t* status = nullptr;
-    uint64_t* ticket = nullptr;
-    uint32_t* err = nullptr;
-    unsigned long long* counters = nullptr;
-    uint32_t* tile_sums = nullptr;
-    uint64_t launches = 0;
-};
+using gps::fail;
+using gps::hip_fail;
 
 namespace {
 
@@ -113,10 +106,7 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
         HIP_TRY(gps::launch_rts_u32(h->shape, inclusive, in, out, n, tiles, vec_ok, h->tile_sums, s));
         return GPS_OK;
     }
-    if (h->launches >= kLaunchesBeforeReset) {
-        HIP_TRY(gps::launch_reset_status(h->status, h->max_tiles, h->ticket, s));
-        h->launches = 0;
-    }
+    HIP_TRY(gps::before_epoch_launch(h, s));
     gps::ScanParams p{};
     p.in = in;
     p.out = out;
@@ -136,7 +126,6 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
     } else {
         HIP_TRY(gps::launch_csdl_u32(h->shape, inclusive, p, s));
     }
-    ++h->launches;
     return GPS_OK;
 }
 
@@ -173,8 +162,8 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     if (stall_period == 1) return fail(GPS_ERR_INVALID, "stall_period must be 0 or >= 2");
     if (stall_period && algo != GPS_ALGO_CSDLDF)
         return fail(GPS_ERR_INVALID, "stall emulation needs the decoupled-fallback algorithm");
-    const uint64_t tiles = (max_n + shape.tile() - 1) / shape.tile();
-    if (tiles >= (1ull << 32) - 1) return fail(GPS_ERR_TOO_LARGE, "max_n too large");
+    const uint64_t tiles = (max_n + kMinTileElems - 1) / kMinTileElems;
+    if (tiles >= (1ull << 31)) return fail(GPS_ERR_TOO_LARGE, "max_n too large");
 
     gps_handle* h = new gps_handle();
     h->algo = algo;

@@ -1,0 +1,133 @@
+// C ABI of the scan forms beside the plain uint32 scan: batched segmented
+// scan (one launch, graph-capturable) and its per-segment-launch baseline.
+#include <hip/hip_runtime.h>
+
+#include <new>
+
+#include "handle.hpp"
+
+using gps::fail;
+using gps::hip_fail;
+
+namespace {
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Segmented kernels use the handle's tile shape when it is one of theirs,
+// otherwise 1024 threads x 4 uint4 (16384-element tiles).
+gps::TileShape seg_shape(const gps_handle* h) {
+    return gps::seg_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
+}
+
+int check_seg(gps_handle* h, const void* in, const void* out, const void* offsets, uint64_t n) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "segmented scans need a CSDL handle");
+    if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
+    if (!offsets) return fail(GPS_ERR_INVALID, "null offsets");
+    if (n > h->max_n)
+        return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
+                    (unsigned long long)h->max_n);
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 3u)
+        return fail(GPS_ERR_INVALID, "uint32 buffers must be 4-byte aligned");
+    if (reinterpret_cast<uintptr_t>(offsets) & 7u) return fail(GPS_ERR_INVALID, "offsets must be 8-byte aligned");
+    return GPS_OK;
+}
+
+int enqueue_segmented(gps_handle* h, const uint32_t* in, uint32_t* out, const uint64_t* offsets,
+                      uint64_t num_segments, uint64_t n, bool inclusive, hipStream_t s) {
+    if (n == 0) return GPS_OK;
+    HIP_TRY(gps::before_epoch_launch(h, s));
+    HIP_TRY(gps::launch_segscan_u32(seg_shape(h), inclusive, in, out, offsets, num_segments, n,
+                                    aligned16(in) && aligned16(out), h->status, h->ticket, h->err, s));
+    return GPS_OK;
+}
+
+}  // namespace
+
+struct gps_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int gps_segmented_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, const uint64_t* d_offsets,
+                           uint64_t num_segments, uint64_t n, int inclusive, void* stream) {
+    int rc = check_seg(h, d_in, d_out, d_offsets, n);
+    if (rc) return rc;
+    return enqueue_segmented(h, d_in, d_out, d_offsets, num_segments, n, inclusive != 0,
+                             static_cast<hipStream_t>(stream));
+}
+
+int gps_segmented_scan_u32_per_segment(gps_handle* h, const uint32_t* d_in, uint32_t* d_out,
+                                       const uint64_t* h_offsets, uint64_t num_segments, int inclusive,
+                                       void* stream) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (h->algo == GPS_ALGO_RTS) return fail(GPS_ERR_UNSUPPORTED, "per-segment loop needs a CSDL handle");
+    if (!h_offsets) return fail(GPS_ERR_INVALID, "null offsets");
+    for (uint64_t s = 0; s < num_segments; ++s) {
+        const uint64_t a = h_offsets[s], b = h_offsets[s + 1];
+        if (b < a) return fail(GPS_ERR_INVALID, "offsets must be non-decreasing (segment %llu)", (unsigned long long)s);
+        if (b == a) continue;
+        const int rc = inclusive ? gps_inclusive_scan_u32(h, d_in + a, d_out + a, b - a, stream)
+                                 : gps_exclusive_scan_u32(h, d_in + a, d_out + a, b - a, stream);
+        if (rc) return rc;
+    }
+    return GPS_OK;
+}
+
+int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, const uint64_t* d_offsets,
+                               uint64_t num_segments, uint64_t n, int inclusive, gps_graph** out) {
+    if (!out) return fail(GPS_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = check_seg(h, d_in, d_out, d_offsets, n);
+    if (rc) return rc;
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    gps_graph* g = new (std::nothrow) gps_graph();
+    if (!g) {
+        hipStreamDestroy(s);
+        return fail(GPS_ERR_INVALID, "out of host memory");
+    }
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        // No re-arm check inside a capture: a graph node must not depend on
+        // the host-side launch count (gps_reset re-arms explicitly).
+        rc = n ? (gps::launch_segscan_u32(seg_shape(h), inclusive != 0, d_in, d_out, d_offsets, num_segments, n,
+                                          aligned16(d_in) && aligned16(d_out), h->status, h->ticket, h->err, s) ==
+                          hipSuccess
+                      ? GPS_OK
+                      : GPS_ERR_HIP)
+               : GPS_OK;
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(s, &graph);
+        g->graph = graph;
+        if (rc != GPS_OK && e == hipSuccess) e = hipErrorLaunchFailure;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    hipStreamDestroy(s);
+    if (e != hipSuccess) {
+        if (g->exec) hipGraphExecDestroy(g->exec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        delete g;
+        return hip_fail(e, "segmented graph capture");
+    }
+    *out = g;
+    return GPS_OK;
+}
+
+int gps_graph_launch(gps_graph* g, void* stream) {
+    if (!g || !g->exec) return fail(GPS_ERR_INVALID, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    return GPS_OK;
+}
+
+int gps_graph_destroy(gps_graph* g) {
+    if (!g) return fail(GPS_ERR_INVALID, "null graph");
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return GPS_OK;
+}
+
+}  // extern "C"

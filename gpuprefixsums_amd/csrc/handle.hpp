@@ -1,0 +1,46 @@
+// The C-ABI handle and the error helpers shared by the entry-point files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gps.h"
+#include "gps_internal.hpp"
+
+// Smallest tile of any kernel shape (256 threads x 2 uint4): status words are
+// sized for it so every entry point of a handle, whatever shape it runs, has
+// a word per tile.
+constexpr uint32_t kMinTileElems = 2048;
+
+struct gps_handle {
+    int algo = 0;
+    uint64_t max_n = 0;
+    gps::TileShape shape{256, 4};
+    uint32_t spin_limit = 4;
+    uint32_t stall_period = 0;
+    uint32_t schedule = GPS_SCHEDULE_TILE;
+    uint32_t max_tiles = 0;     // status words allocated (for kMinTileElems tiles)
+    int device = 0;
+    uint64_t* status = nullptr;
+    uint64_t* ticket = nullptr;
+    uint32_t* err = nullptr;
+    unsigned long long* counters = nullptr;
+    uint32_t* tile_sums = nullptr;  // reduce-then-scan spine
+    uint64_t launches = 0;          // epoch-tagged launches since the last re-arm
+};
+
+namespace gps {
+
+// Records the message for gps_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+// Re-arms the status words when the 30-bit epoch would wrap; counts a launch.
+hipError_t before_epoch_launch(gps_handle* h, hipStream_t s);
+
+}  // namespace gps
+
+#define HIP_TRY(expr)                                          \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return gps::hip_fail(e_, #expr); \
+    } while (0)

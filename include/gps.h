@@ -88,6 +88,32 @@ int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems);
 int gps_inclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 
+/* ---- batched segmented scan (no reference counterpart) ------------------- */
+
+/* Scans num_segments segments stored back to back: segment s is
+ * [d_offsets[s], d_offsets[s+1]) of d_in / d_out, and the running sum restarts
+ * at every segment start (position 0 always starts one). d_offsets is a
+ * device array of num_segments + 1 non-decreasing uint64 offsets with
+ * d_offsets[num_segments] == n; empty segments are allowed. Exclusive output
+ * puts 0 at each segment start. One kernel launch, no allocation and no
+ * memset, so it may be captured into a graph. Needs a GPS_ALGO_CSDL handle. */
+int gps_segmented_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, const uint64_t* d_offsets,
+                           uint64_t num_segments, uint64_t n, int inclusive, void* stream);
+/* The same result with one plain scan launch per non-empty segment, from
+ * host-side offsets (the batched scan's baseline). */
+int gps_segmented_scan_u32_per_segment(gps_handle* h, const uint32_t* d_in, uint32_t* d_out,
+                                       const uint64_t* h_offsets, uint64_t num_segments, int inclusive,
+                                       void* stream);
+
+/* A hipGraph holding one gps_segmented_scan_u32 call with fixed arguments
+ * (captured on a private stream). Replays reuse the handle's status words
+ * through their epochs; after 2^30 - 1 replays call gps_reset first. */
+typedef struct gps_graph gps_graph;
+int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, const uint64_t* d_offsets,
+                               uint64_t num_segments, uint64_t n, int inclusive, gps_graph** out);
+int gps_graph_launch(gps_graph* g, void* stream);
+int gps_graph_destroy(gps_graph* g);
+
 /* CSDLDF fallback counters accumulated since the last reset:
  * [0] status polls, [1] fallbacks (redundant tile reductions), [2] successful
  * CAS insertions, [3] emulated stalls that hit their watchdog.
