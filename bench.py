@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
-ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "n2p20"]
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "n2p20", "segmented"]
 
 
 def log(msg: str) -> None:
@@ -113,6 +113,48 @@ class Legs:
         e1.record(st)
         torch.cuda.synchronize()
         out["copy_ceiling_gbps"] = round(gbps(self.n, e0.elapsed_time(e1) / 20), 1)
+
+    def segmented(self, out):
+        """4096 segments, log-normal lengths (median 4096, sigma 1.5, clipped to
+        [1, 2^22]), values uniform in [0, 1023], seed 3, exclusive: one launch
+        replayed from a hipGraph vs one plain scan launch per segment."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        S = 4096
+        off = oracle.gen_lognormal_offsets(3, S, median=4096.0, sigma=1.5, lo=1, hi=1 << 22)
+        n = int(off[-1])
+        x = torch.from_numpy(oracle.gen_u32(3, n, 10).view(np.int32)).to("cuda")
+        d_off = torch.from_numpy(off.view(np.int64)).to("cuda")
+        a = torch.empty_like(x)
+        b = torch.empty_like(x)
+        st = torch.cuda.current_stream()
+        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+        with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+            with h.segmented_graph(x, a, d_off, S, n, False) as g:
+                for _ in range(3):
+                    g.launch()
+                e0, e1 = ev(), ev()
+                e0.record(st)
+                for _ in range(20):
+                    g.launch()
+                e1.record(st)
+                torch.cuda.synchronize()
+                graph_ms = e0.elapsed_time(e1) / 20
+            h.segmented_per_segment(x, b, off, False)
+            torch.cuda.synchronize()
+            e0, e1 = ev(), ev()
+            e0.record(st)
+            for _ in range(3):
+                h.segmented_per_segment(x, b, off, False)
+            e1.record(st)
+            torch.cuda.synchronize()
+            loop_ms = e0.elapsed_time(e1) / 3
+            if not torch.equal(a, b) or h.device_error():
+                raise RuntimeError("segmented outputs differ")
+        nbytes = 8 * n + 8 * (S + 1)
+        out["segmented"] = {
+            "segments": S, "n_total": n, "graph_ms": round(graph_ms, 4), "per_segment_launches_ms": round(loop_ms, 3),
+            "speedup": round(loop_ms / graph_ms, 1), "graph_gbps": round(nbytes / graph_ms / 1e6, 1),
+            "per_segment_gbps": round(nbytes / loop_ms / 1e6, 1), "bytes": nbytes}
 
     def n2p20(self, out):
         gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np

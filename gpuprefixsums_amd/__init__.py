@@ -117,6 +117,19 @@ class ScanHandle:
                                        _stream(stream), iters, ms))
         return list(ms)
 
+    def inclusive_f32(self, d_in, d_out, n: Optional[int] = None, compensated: bool = True, stream=None) -> None:
+        """FP32 -> FP32 inclusive scan (two-sum compensated carries unless compensated=False)."""
+        n = _numel(d_in) if n is None else n
+        check(load().gps_inclusive_scan_f32(self.handle, _ptr(d_in), _ptr(d_out), n, int(compensated),
+                                            _stream(stream)))
+
+    def inclusive_bf16_f32(self, d_in, d_out, n: Optional[int] = None, compensated: bool = True,
+                           stream=None) -> None:
+        """BF16 (uint16 bits) -> FP32 inclusive scan."""
+        n = _numel(d_in) if n is None else n
+        check(load().gps_inclusive_scan_bf16_f32(self.handle, _ptr(d_in), _ptr(d_out), n, int(compensated),
+                                                 _stream(stream)))
+
     def segmented(self, d_in, d_out, d_offsets, num_segments: int, n: int, inclusive: bool = False,
                   stream=None) -> None:
         """Batched segmented scan in one launch (d_offsets: device uint64[num_segments + 1])."""

@@ -62,6 +62,8 @@ SIGNATURES = {
     "gps_segmented_scan_u32": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, _vp]),
     "gps_segmented_scan_u32_per_segment": (_int, [_vp, _vp, _vp, ctypes.POINTER(_u64), _u64, _int, _vp]),
     "gps_segmented_graph_create": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, ctypes.POINTER(_vp)]),
+    "gps_inclusive_scan_f32": (_int, [_vp, _vp, _vp, _u64, _int, _vp]),
+    "gps_inclusive_scan_bf16_f32": (_int, [_vp, _vp, _vp, _u64, _int, _vp]),
     "gps_graph_launch": (_int, [_vp, _vp]),
     "gps_graph_destroy": (_int, [_vp]),
     "gps_get_counters": (_int, [_vp, ctypes.POINTER(_u64)]),

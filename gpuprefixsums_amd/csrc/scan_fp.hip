@@ -1,0 +1,309 @@
+// Floating-point inclusive scans (FP32 or BF16 input, FP32 output) by chained
+// scan with decoupled lookback, with optional two-sum compensated carries.
+//
+// No reference counterpart (a north-star extension); the chained-scan
+// structure is the uint32 one (GPUPrefixSumsCUDA/ChainedScanDecoupledLookback.cuh:79-153).
+//
+// Compensated mode carries every partial sum above the element level as an
+// unevaluated pair (s, c), value s + c, combined with Knuth's TwoSum (the
+// exact rounding error of s1 + s2) and Neumaier's accumulation of the error
+// terms: lane totals, the DPP wave scan, the run carry across vectors, wave
+// totals, tile aggregates, and the inter-tile lookback chain. Each output is
+// recomputed from its input with a pair carry and rounded once to FP32, so
+// its error is about one FP32 rounding of the exact prefix, independent of n.
+// The plain mode is the same kernel with single floats (c == 0).
+//
+// This file is compiled with -ffp-contract=off: a fused multiply-add would
+// break the TwoSum error terms.
+#include "gps_internal.hpp"
+#include "tile_u32.hpp"
+
+namespace gps {
+
+struct ff {
+    float s, c;
+};
+
+__device__ __forceinline__ ff two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+
+template <bool COMP>
+__device__ __forceinline__ ff ff_add(ff x, ff y) {
+    if (!COMP) return {x.s + y.s, 0.f};
+    ff t = two_sum(x.s, y.s);
+    t.c += x.c + y.c;
+    return t;
+}
+
+template <bool COMP>
+__device__ __forceinline__ ff ff_add1(ff x, float v) {
+    if (!COMP) return {x.s + v, 0.f};
+    ff t = two_sum(x.s, v);
+    t.c += x.c;
+    return t;
+}
+
+// DPP move of a float (controls and masks must be constants).
+#define GPS_DPP_F(x, CTRL, RM, BC) \
+    __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, (x)), CTRL, RM, 0xf, BC))
+
+// Inclusive wave scan of pairs (same DPP schedule as the integer scan).
+template <bool COMP>
+__device__ __forceinline__ ff wave_incl_scan_ff(ff x) {
+#define GPS_FF_STEP(CTRL, RM, BC)                                                 \
+    {                                                                             \
+        const ff y = {GPS_DPP_F(x.s, CTRL, RM, BC), COMP ? GPS_DPP_F(x.c, CTRL, RM, BC) : 0.f}; \
+        x = ff_add<COMP>(y, x);                                                   \
+    }
+    GPS_FF_STEP(0x111, 0xf, true)
+    GPS_FF_STEP(0x112, 0xf, true)
+    GPS_FF_STEP(0x114, 0xf, true)
+    GPS_FF_STEP(0x118, 0xf, true)
+    GPS_FF_STEP(0x142, 0xa, false)
+    GPS_FF_STEP(0x143, 0xc, false)
+#undef GPS_FF_STEP
+    return x;
+}
+
+__device__ __forceinline__ float readlane_f(float x, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+}
+
+// Status pair for the FP scans: two 64-bit words per tile, both carrying the
+// flag and epoch (same layout as the integer words, value = float bits):
+// status[2t] = s, status[2t+1] = c. Readers accept a tile only when both words
+// are current and agree on the flag; each flag is written once per launch, so
+// agreeing words belong to the same publication.
+template <bool COMP>
+__device__ __forceinline__ void publish_ff(uint64_t* status, uint32_t tile, ff v, uint64_t flag, uint32_t epoch) {
+    atomic_store_agent(status + 2ull * tile, make_status(__builtin_bit_cast(uint32_t, v.s), flag, epoch));
+    atomic_store_agent(status + 2ull * tile + 1, make_status(COMP ? __builtin_bit_cast(uint32_t, v.c) : 0u, flag, epoch));
+}
+
+// Window lookback over pair words (one full wave); returns the exclusive
+// prefix of `tile` as a pair.
+template <bool COMP>
+__device__ __forceinline__ ff lookback_ff(const uint64_t* status, uint32_t tile, uint32_t epoch, uint32_t* err) {
+    const uint32_t lane = lane_id();
+    ff prefix = {0.f, 0.f};
+    int64_t top = (int64_t)tile - 1;
+    uint32_t empty_polls = 0;
+    while (true) {
+        const int64_t idx = top - (int64_t)lane;
+        uint64_t ws = make_status(0u, kFlagIncl, epoch), wc = ws;
+        if (idx >= 0) {
+            ws = atomic_load_agent(status + 2 * idx);
+            wc = atomic_load_agent(status + 2 * idx + 1);
+        }
+        const uint32_t fs = status_flag(ws, epoch), fc = status_flag(wc, epoch);
+        const uint32_t f = fs == fc ? fs : 0u;
+        const uint64_t ready = ballot(f != 0u);
+        const uint64_t incl = ballot(f == (uint32_t)kFlagIncl);
+        const uint32_t run = (~ready) ? (uint32_t)__builtin_ctzll(~ready) : 64u;
+        const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+        const uint32_t take = first_incl < run ? first_incl + 1u : run;
+        if (take) {
+            ff v = {0.f, 0.f};
+            if (lane < take) v = {__builtin_bit_cast(float, (uint32_t)ws), __builtin_bit_cast(float, (uint32_t)wc)};
+            const ff s = wave_incl_scan_ff<COMP>(v);
+            const ff tot = {readlane_f(s.s, 63), COMP ? readlane_f(s.c, 63) : 0.f};
+            prefix = ff_add<COMP>(tot, prefix);
+            if (first_incl < run) return prefix;
+            top -= run;
+            empty_polls = 0;
+        } else {
+            if (watchdog_expired(empty_polls, err)) {
+                if (lane == 0) raise_device_error(err, kErrLookbackWatchdog);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+struct FpParams {
+    const void* in;  // float or bf16 bits
+    float* out;
+    uint64_t n;
+    uint32_t num_tiles;
+    uint32_t vec_ok;
+    uint64_t* status;  // 2 words per tile
+    uint64_t* ticket;
+    uint32_t* err;
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+
+template <bool BF16, int VEC>
+__device__ __forceinline__ void load_fp(f32x4 (&x)[VEC], const void* in, uint64_t base, uint64_t n, bool vec_path) {
+    if (vec_path) {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            if (BF16) {
+                const u16x4 b = __builtin_nontemporal_load(reinterpret_cast<const u16x4*>(
+                    static_cast<const uint16_t*>(in) + base + (uint64_t)k * 256u));
+#pragma unroll
+                for (int c = 0; c < 4; ++c) x[k][c] = __builtin_bit_cast(float, (uint32_t)b[c] << 16);
+            } else {
+                x[k] = __builtin_nontemporal_load(
+                    reinterpret_cast<const f32x4*>(static_cast<const float*>(in) + base + (uint64_t)k * 256u));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                float v = 0.f;
+                if (i < n) {
+                    if (BF16)
+                        v = __builtin_bit_cast(float, (uint32_t)static_cast<const uint16_t*>(in)[i] << 16);
+                    else
+                        v = static_cast<const float*>(in)[i];
+                }
+                x[k][c] = v;
+            }
+        }
+    }
+}
+
+template <int BLOCK, int VEC, bool BF16, bool COMP>
+__global__ __launch_bounds__(BLOCK) void scan_fp(FpParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ float s_ws[C::kWaves], s_wc[C::kWaves];
+    __shared__ uint32_t s_ticket[2];
+    __shared__ float s_ps, s_pc;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, p.num_tiles);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    __syncthreads();
+    const uint32_t tile = s_ticket[0], epoch = s_ticket[1];
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    const bool vec_path = p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n);
+    f32x4 x[VEC];
+    load_fp<BF16, VEC>(x, p.in, base, p.n, vec_path);
+
+    // Per vector: lane total (pair), wave exclusive scan -> lane carry L_k.
+    ff lane_carry[VEC];
+    ff run = {0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        ff t = {x[k].x, 0.f};
+        t = ff_add1<COMP>(t, x[k].y);
+        t = ff_add1<COMP>(t, x[k].z);
+        t = ff_add1<COMP>(t, x[k].w);
+        const ff s = wave_incl_scan_ff<COMP>(t);
+        // exclusive = inclusive of the lane below (lane 0: zero)
+        ff e;
+        e.s = GPS_DPP_F(s.s, 0x138, 0xf, true);  // wave_shr:1
+        e.c = COMP ? GPS_DPP_F(s.c, 0x138, 0xf, true) : 0.f;
+        lane_carry[k] = ff_add<COMP>(run, e);
+        const ff tot = {readlane_f(s.s, 63), COMP ? readlane_f(s.c, 63) : 0.f};
+        run = ff_add<COMP>(run, tot);
+    }
+    if (lane == 0) {
+        s_ws[wave] = run.s;
+        s_wc[wave] = run.c;
+    }
+    __syncthreads();
+    ff wave_pre = {0.f, 0.f}, tile_tot = {0.f, 0.f};
+#pragma unroll
+    for (uint32_t q = 0; q < C::kWaves; ++q) {
+        const ff v = {s_ws[q], s_wc[q]};
+        if (q < wave) wave_pre = ff_add<COMP>(wave_pre, v);
+        tile_tot = ff_add<COMP>(tile_tot, v);
+    }
+    if (wave == 0) {
+        ff prefix = {0.f, 0.f};
+        if (tile == 0) {
+            if (lane == 0) publish_ff<COMP>(p.status, 0, tile_tot, kFlagIncl, epoch);
+        } else {
+            if (lane == 0) publish_ff<COMP>(p.status, tile, tile_tot, kFlagAgg, epoch);
+            prefix = lookback_ff<COMP>(p.status, tile, epoch, p.err);
+            if (lane == 0) publish_ff<COMP>(p.status, tile, ff_add<COMP>(prefix, tile_tot), kFlagIncl, epoch);
+        }
+        if (lane == 0) {
+            s_ps = prefix.s;
+            s_pc = prefix.c;
+        }
+    }
+    __syncthreads();
+    const ff base_carry = ff_add<COMP>({s_ps, s_pc}, wave_pre);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        ff acc = ff_add<COMP>(base_carry, lane_carry[k]);
+        f32x4 y;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            acc = ff_add1<COMP>(acc, x[k][c]);
+            y[c] = acc.s + acc.c;
+        }
+        x[k] = y;
+    }
+    if (vec_path) {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k)
+            __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
+    } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                if (i < p.n) p.out[i] = x[k][c];
+            }
+    }
+}
+
+#define GPS_FP_SHAPES(X) \
+    X(256, 4)            \
+    X(1024, 4)           \
+    X(1024, 8)
+
+bool fp_shape_supported(TileShape s) {
+#define X(B, V) \
+    if (s.block == B && s.vec == V) return true;
+    GPS_FP_SHAPES(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_scan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out, uint64_t n,
+                          uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream) {
+    FpParams p{};
+    p.in = in;
+    p.out = out;
+    p.n = n;
+    p.num_tiles = (uint32_t)((n + s.tile() - 1) / s.tile());
+    p.vec_ok = vec_ok;
+    p.status = status;
+    p.ticket = ticket;
+    p.err = err;
+#define X(B, V)                                                                                                 \
+    if (s.block == B && s.vec == V) {                                                                           \
+        if (bf16 && compensated)                                                                                \
+            hipLaunchKernelGGL((scan_fp<B, V, true, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);       \
+        else if (bf16)                                                                                          \
+            hipLaunchKernelGGL((scan_fp<B, V, true, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);      \
+        else if (compensated)                                                                                   \
+            hipLaunchKernelGGL((scan_fp<B, V, false, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);      \
+        else                                                                                                    \
+            hipLaunchKernelGGL((scan_fp<B, V, false, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);     \
+        return hipGetLastError();                                                                               \
+    }
+    GPS_FP_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gps

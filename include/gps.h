@@ -114,6 +114,18 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
 int gps_graph_launch(gps_graph* g, void* stream);
 int gps_graph_destroy(gps_graph* g);
 
+/* ---- floating-point inclusive scans (no reference counterpart) ----------- */
+
+/* FP32 or BF16 (raw uint16 bit patterns) input, FP32 output, accumulated in
+ * FP32. compensated != 0 carries every partial sum above the element level
+ * (lane, wave, tile and the inter-tile lookback chain) as a two-float pair
+ * combined with TwoSum, and rounds each output once; compensated == 0 is the
+ * plain FP32 chained scan. One kernel launch; needs a GPS_ALGO_CSDL handle. */
+int gps_inclusive_scan_f32(gps_handle* h, const float* d_in, float* d_out, uint64_t n, int compensated,
+                           void* stream);
+int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, uint64_t n, int compensated,
+                                void* stream);
+
 /* CSDLDF fallback counters accumulated since the last reset:
  * [0] status polls, [1] fallbacks (redundant tile reductions), [2] successful
  * CAS insertions, [3] emulated stalls that hit their watchdog.
