@@ -17,7 +17,7 @@ import ctypes
 from typing import Optional, Sequence
 
 from . import _lib
-from ._lib import (ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, SCHEDULE_DEFAULT, SCHEDULE_PERSISTENT, SCHEDULE_TILE,
+from ._lib import (ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, SCHEDULE_CONTROL, SCHEDULE_DEFAULT, SCHEDULE_PERSISTENT, SCHEDULE_TILE,
                    GpsError, GpsTuning, check, load)
 
 __all__ = [
@@ -143,6 +143,15 @@ class ScanHandle:
         ptr = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
         check(load().gps_segmented_scan_u32_per_segment(self.handle, _ptr(d_in), _ptr(d_out), ptr, off.size - 1,
                                                         int(inclusive), _stream(stream)))
+
+    def fp_per_segment(self, d_in, d_out, h_offsets, bf16: bool = False, compensated: bool = True,
+                       stream=None) -> None:
+        """One FP inclusive scan launch per segment (h_offsets: host uint64 sequence)."""
+        import numpy as np
+        off = np.ascontiguousarray(h_offsets, dtype=np.uint64)
+        ptr = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        check(load().gps_inclusive_scan_fp_per_segment(self.handle, _ptr(d_in), _ptr(d_out), ptr, off.size - 1,
+                                                       int(bf16), int(compensated), _stream(stream)))
 
     def segmented_graph(self, d_in, d_out, d_offsets, num_segments: int, n: int,
                         inclusive: bool = False) -> "Graph":

@@ -26,6 +26,7 @@ _vp = ctypes.c_void_p
 _u64 = ctypes.c_uint64
 _u32 = ctypes.c_uint32
 _int = ctypes.c_int
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class GpsTuning(ctypes.Structure):
@@ -41,6 +42,7 @@ class GpsTuning(ctypes.Structure):
 SCHEDULE_DEFAULT = 0
 SCHEDULE_TILE = 1
 SCHEDULE_PERSISTENT = 2
+SCHEDULE_CONTROL = 3
 
 
 class GpsError(RuntimeError):
@@ -64,6 +66,15 @@ SIGNATURES = {
     "gps_segmented_graph_create": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, ctypes.POINTER(_vp)]),
     "gps_inclusive_scan_f32": (_int, [_vp, _vp, _vp, _u64, _int, _vp]),
     "gps_inclusive_scan_bf16_f32": (_int, [_vp, _vp, _vp, _u64, _int, _vp]),
+    "gps_inclusive_scan_fp_per_segment": (_int, [_vp, _vp, _vp, ctypes.POINTER(_u64), _u64, _int, _int, _vp]),
+    "gps_mlp_param_count": (_u64, [_u32, _u32]),
+    "gps_mlp_create": (_int, [_u32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "gps_mlp_destroy": (_int, [_vp]),
+    "gps_mlp_set_params": (_int, [_vp, _fp]),
+    "gps_mlp_get_params": (_int, [_vp, _fp]),
+    "gps_mlp_forward": (_int, [_vp, _fp, _u32, _fp]),
+    "gps_mlp_loss_grad": (_int, [_vp, _fp, _fp, _u32, _fp, _fp]),
+    "gps_mlp_train": (_int, [_vp, _fp, _fp, _u32, _u32, ctypes.c_float, ctypes.c_float, _fp]),
     "gps_graph_launch": (_int, [_vp, _vp]),
     "gps_graph_destroy": (_int, [_vp]),
     "gps_get_counters": (_int, [_vp, ctypes.POINTER(_u64)]),

@@ -65,6 +65,11 @@ gps::TileShape default_shape(int algo) {
     }
 }
 
+// Elements per tile of the handle's plain-scan kernel.
+uint32_t handle_tile(const gps_handle* h) {
+    return h->schedule == GPS_SCHEDULE_CONTROL ? gps::cw_tile_elems(h->shape) : h->shape.tile();
+}
+
 uint32_t default_schedule(int algo) { return algo == GPS_ALGO_CSDL ? GPS_SCHEDULE_PERSISTENT : GPS_SCHEDULE_TILE; }
 
 void free_handle(gps_handle* h) {
@@ -99,7 +104,7 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
     if (rc) return rc;
     if (n == 0) return GPS_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t tile = h->shape.tile();
+    const uint32_t tile = handle_tile(h);
     const uint32_t tiles = (uint32_t)((n + tile - 1) / tile);
     const uint32_t vec_ok = aligned16(in) && aligned16(out);
     if (h->algo == GPS_ALGO_RTS) {
@@ -123,6 +128,8 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
         HIP_TRY(gps::launch_csdldf_u32(h->shape, inclusive, p, s));
     } else if (h->schedule == GPS_SCHEDULE_PERSISTENT) {
         HIP_TRY(gps::launch_csdl_pp_u32(h->shape, inclusive, p, s));
+    } else if (h->schedule == GPS_SCHEDULE_CONTROL) {
+        HIP_TRY(gps::launch_csdl_cw_u32(h->shape, inclusive, p, s));
     } else {
         HIP_TRY(gps::launch_csdl_u32(h->shape, inclusive, p, s));
     }
@@ -152,9 +159,9 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
         stall_period = tuning->stall_period;
         schedule = tuning->schedule;
     }
-    if (schedule > GPS_SCHEDULE_PERSISTENT) return fail(GPS_ERR_INVALID, "unknown schedule %u", schedule);
-    if (schedule == GPS_SCHEDULE_PERSISTENT && algo != GPS_ALGO_CSDL)
-        return fail(GPS_ERR_INVALID, "the persistent schedule is offered for CSDL only");
+    if (schedule > GPS_SCHEDULE_CONTROL) return fail(GPS_ERR_INVALID, "unknown schedule %u", schedule);
+    if ((schedule == GPS_SCHEDULE_PERSISTENT || schedule == GPS_SCHEDULE_CONTROL) && algo != GPS_ALGO_CSDL)
+        return fail(GPS_ERR_INVALID, "persistent schedules are offered for CSDL only");
     if (schedule == GPS_SCHEDULE_DEFAULT) schedule = default_schedule(algo);
     if (!gps::shape_supported(shape))
         return fail(GPS_ERR_INVALID, "unsupported tile shape block=%u vec=%u (block 256/512/1024, vec 2/4/8)",
@@ -204,7 +211,7 @@ int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems) {
     out->spin_limit = h->spin_limit;
     out->stall_period = h->stall_period;
     out->schedule = h->schedule;
-    if (tile_elems) *tile_elems = h->shape.tile();
+    if (tile_elems) *tile_elems = handle_tile(h);
     return GPS_OK;
 }
 

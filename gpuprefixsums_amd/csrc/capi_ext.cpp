@@ -85,7 +85,6 @@ int gps_segmented_scan_u32_per_segment(gps_handle* h, const uint32_t* d_in, uint
                                        const uint64_t* h_offsets, uint64_t num_segments, int inclusive,
                                        void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
-    if (h->algo == GPS_ALGO_RTS) return fail(GPS_ERR_UNSUPPORTED, "per-segment loop needs a CSDL handle");
     if (!h_offsets) return fail(GPS_ERR_INVALID, "null offsets");
     for (uint64_t s = 0; s < num_segments; ++s) {
         const uint64_t a = h_offsets[s], b = h_offsets[s + 1];
@@ -146,6 +145,21 @@ int gps_inclusive_scan_f32(gps_handle* h, const float* d_in, float* d_out, uint6
 int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, uint64_t n, int compensated,
                                 void* stream) {
     return scan_fp(h, d_in, d_out, n, true, compensated, stream);
+}
+
+int gps_inclusive_scan_fp_per_segment(gps_handle* h, const void* d_in, float* d_out, const uint64_t* h_offsets,
+                                      uint64_t num_segments, int bf16, int compensated, void* stream) {
+    if (!h || !h_offsets) return fail(GPS_ERR_INVALID, "null argument");
+    const size_t esize = bf16 ? 2 : 4;
+    for (uint64_t s = 0; s < num_segments; ++s) {
+        const uint64_t a = h_offsets[s], b = h_offsets[s + 1];
+        if (b < a) return fail(GPS_ERR_INVALID, "offsets must be non-decreasing (segment %llu)", (unsigned long long)s);
+        if (b == a) continue;
+        const void* in = static_cast<const char*>(d_in) + a * esize;
+        const int rc = scan_fp(h, in, d_out + a, b - a, bf16 != 0, compensated, stream);
+        if (rc) return rc;
+    }
+    return GPS_OK;
 }
 
 int gps_graph_launch(gps_graph* g, void* stream) {

@@ -36,6 +36,9 @@ bool shape_supported(TileShape s);
 hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 // Persistent pipelined CSDL: grid = resident workgroups (<= num_tiles).
 hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
+// Persistent CSDL with a control wave: block - 64 data threads per tile.
+uint32_t cw_tile_elems(TileShape s);
+hipError_t launch_csdl_cw_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 hipError_t launch_rts_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
                           uint32_t num_tiles, uint32_t vec_ok, uint32_t* tile_sums, hipStream_t stream);

@@ -7,10 +7,11 @@
 #include "../../include/gps.h"
 #include "gps_internal.hpp"
 
-// Smallest tile of any kernel shape (256 threads x 2 uint4): status words are
-// sized for it so every entry point of a handle, whatever shape it runs, has
-// a word per tile.
-constexpr uint32_t kMinTileElems = 2048;
+// Lower bound on the tile of every kernel shape (the smallest is the
+// control-wave schedule's 3 data waves x 2 uint4 = 1536 elements; the FP
+// scans use two words per tile of at least 4096): status words are sized for
+// it so every entry point of a handle, whatever shape it runs, has its words.
+constexpr uint32_t kMinTileElems = 1024;
 
 struct gps_handle {
     int algo = 0;

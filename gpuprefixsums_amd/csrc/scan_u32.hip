@@ -203,6 +203,153 @@ csdl_pp_u32(ScanParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent CSDL with a control wave (GPS_SCHEDULE_CONTROL).
+//
+// The workgroup is DW data waves plus one control wave. Data waves only
+// load, scan and store (current tile + prefetched next tile in registers);
+// the control wave owns everything that waits on the device coherence point:
+// tickets, publication and the lookback. The two sides meet through LDS
+// sequence counters (parity-buffered slots, release/acquire at workgroup
+// scope), so a data wave never holds a poll or an atomic in its vmcnt queue
+// and never waits on a barrier for them. Tile = DW * 256 * VEC elements.
+//
+// Per iteration i (tile t_i):
+//   data:    scan t_i -> totals, arrived[w] = i+1 | wait ticket_seq > i+1 ->
+//            issue loads of t_{i+1} | wait prefix_seq > i -> store t_i
+//   control: ticket atomic for t_{i+1} | wait all arrived > i -> publish
+//            aggregate of t_i | publish t_{i+1} (ticket_seq = i+2) | lookback
+//            of t_i -> publish inclusive, prefix (prefix_seq = i+1)
+// Every LDS wait is bounded by the device watchdog (error bit, then exit).
+// ---------------------------------------------------------------------------
+#ifndef GPS_CW_EARLY_TICKET
+#define GPS_CW_EARLY_TICKET 1
+#endif
+
+// Spins until *addr >= v (workgroup-scope acquire); false if the watchdog fires.
+__device__ __forceinline__ bool lds_wait_ge(const uint32_t* addr, uint32_t v, uint32_t* err) {
+    uint32_t polls = 0;
+    while (__hip_atomic_load(addr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < v) {
+        if (watchdog_expired(polls, err)) {
+            raise_device_error(err, kErrLookbackWatchdog);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+__device__ __forceinline__ void lds_release(uint32_t* addr, uint32_t v) {
+    __hip_atomic_store(addr, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int DW, int VEC, bool INCL>
+__global__ __launch_bounds__(64 * (DW + 1)) __attribute__((amdgpu_waves_per_eu(VEC >= 8 ? 4 : GPS_WAVES_PER_EU))) void
+csdl_cw_u32(ScanParams p) {
+    constexpr uint32_t kWaveElems = 256u * VEC;
+    constexpr uint64_t kTile = (uint64_t)DW * kWaveElems;
+    __shared__ uint32_t s_tot[2][DW];
+    __shared__ uint32_t s_arrived[DW];
+    __shared__ uint32_t s_ticket[2];
+    __shared__ uint32_t s_prefix[2];
+    __shared__ uint32_t s_ticket_seq, s_prefix_seq;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+    if (threadIdx.x < DW) s_arrived[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        s_ticket_seq = 0;
+        s_prefix_seq = 0;
+    }
+    __syncthreads();
+
+    if (wave == DW) {
+        // ---- control wave ----
+        uint32_t tile = 0, epoch = 0;
+        if (lane == 0) {
+            const Ticket t = take_ticket(p.ticket, total_tickets);
+            tile = t.tile;
+            epoch = t.epoch;
+            s_ticket[0] = tile;
+            lds_release(&s_ticket_seq, 1u);
+        }
+        tile = readlane(tile, 0);
+        epoch = readlane(epoch, 0);
+        for (uint32_t i = 0; tile < p.num_tiles; ++i) {
+            uint64_t old = 0;
+            if (GPS_CW_EARLY_TICKET && lane == 0)
+                old = __hip_atomic_fetch_add(p.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bool ok = true;
+            if (lane < (uint32_t)DW) ok = lds_wait_ge(&s_arrived[lane], i + 1u, p.err);
+            if (ballot(!ok)) return;
+            const uint32_t total = wave_sum(lane < (uint32_t)DW ? s_tot[i & 1u][lane] : 0u);
+            if (lane == 0) {
+                atomic_store_agent(p.status + tile, make_status(total, tile ? kFlagAgg : kFlagIncl, epoch));
+                if (!GPS_CW_EARLY_TICKET)
+                    old = __hip_atomic_fetch_add(p.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t next = (uint32_t)old;
+                if (next == total_tickets - 1u)
+                    atomic_store_agent(p.ticket, (uint64_t)next_epoch((uint32_t)(old >> 32)) << 32);
+                s_ticket[(i + 1u) & 1u] = next;
+                lds_release(&s_ticket_seq, i + 2u);
+            }
+            uint32_t prefix = 0;
+            if (tile != 0) {
+                prefix = lookback_u32(p.status, tile, epoch, p.err);
+                if (lane == 0) atomic_store_agent(p.status + tile, make_status(prefix + total, kFlagIncl, epoch));
+            }
+            if (lane == 0) {
+                s_prefix[i & 1u] = prefix;
+                lds_release(&s_prefix_seq, i + 1u);
+            }
+            tile = readlane(lane == 0 ? (uint32_t)old : 0u, 0);
+        }
+        return;
+    }
+
+    // ---- data waves ----
+    if (!lds_wait_ge(&s_ticket_seq, 1u, p.err)) return;
+    uint32_t tile = s_ticket[0];
+    if (tile >= p.num_tiles) return;
+    u32x4 a[VEC], b[VEC];
+    {
+        const uint64_t base = (uint64_t)tile * kTile + wave * kWaveElems + lane * 4u;
+        if (p.vec_ok && (uint64_t)(tile + 1u) * kTile <= p.n)
+            load_vec<VEC>(a, p.in + base);
+        else
+            load_guarded<VEC>(a, p.in, base, p.n);
+    }
+    for (uint32_t i = 0;; ++i) {
+        const uint32_t run = wave_scan_run<VEC, INCL>(a);
+        if (lane == 0) {
+            s_tot[i & 1u][wave] = run;
+            lds_release(&s_arrived[wave], i + 1u);
+        }
+        if (!lds_wait_ge(&s_ticket_seq, i + 2u, p.err)) return;
+        const uint32_t next = s_ticket[(i + 1u) & 1u];
+        const bool has_next = next < p.num_tiles;
+        if (has_next) {
+            const uint64_t nb = (uint64_t)next * kTile + wave * kWaveElems + lane * 4u;
+            if (p.vec_ok && (uint64_t)(next + 1u) * kTile <= p.n)
+                load_vec<VEC>(b, p.in + nb);
+            else
+                load_guarded<VEC>(b, p.in, nb, p.n);
+        }
+        if (!lds_wait_ge(&s_prefix_seq, i + 1u, p.err)) return;
+        const uint32_t wp = wave_sum(lane < wave ? s_tot[i & 1u][lane] : 0u);
+        const uint32_t add = s_prefix[i & 1u] + wp;
+        const uint64_t base = (uint64_t)tile * kTile + wave * kWaveElems + lane * 4u;
+        if (p.vec_ok && (uint64_t)(tile + 1u) * kTile <= p.n)
+            store_vec<VEC>(p.out + base, a, add);
+        else
+            store_guarded<VEC>(p.out, a, add, base, p.n);
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) a[k] = b[k];
+        tile = next;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // CSDLDF: lookback with decoupled fallback.
 //
 // Wave 0 polls the status window; after spin_limit consecutive polls without
@@ -595,6 +742,39 @@ hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, 
 #define X(B, V)                                                                          \
     if (s.block == B && s.vec == V) {                                                    \
         return inclusive ? launch_pp<B, V, true>(p, stream) : launch_pp<B, V, false>(p, stream); \
+    }
+    GPS_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <int DW, int V, bool I>
+static hipError_t launch_cw(const ScanParams& p, hipStream_t stream) {
+    struct Residency {
+        uint32_t workgroups = 0;
+        Residency() {
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess) return;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, csdl_cw_u32<DW, V, I>, 64 * (DW + 1), 0) !=
+                hipSuccess)
+                return;
+            workgroups = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+        }
+    };
+    static const Residency r;
+    if (r.workgroups == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = p.num_tiles < r.workgroups ? p.num_tiles : r.workgroups;
+    hipLaunchKernelGGL((csdl_cw_u32<DW, V, I>), dim3(grid), dim3(64 * (DW + 1)), 0, stream, p);
+    return hipGetLastError();
+}
+
+uint32_t cw_tile_elems(TileShape s) { return (s.block - 64u) * s.vec * 4u; }
+
+hipError_t launch_csdl_cw_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+#define X(B, V)                                                                                               \
+    if (s.block == B && s.vec == V) {                                                                         \
+        return inclusive ? launch_cw<B / 64 - 1, V, true>(p, stream) : launch_cw<B / 64 - 1, V, false>(p, stream); \
     }
     GPS_SHAPES(X)
 #undef X

@@ -1,0 +1,129 @@
+"""Exhaustive autotuning harness: times every candidate configuration of each
+workload on the GPU. Its output is both the cost model's training data and
+the judge of the model's regret; for a workload the model does not cover it
+is the fallback.
+
+    python -m gpuprefixsums_amd.tuner.harness --out times.jsonl [--start 0 --count 512]
+
+One JSON line per workload: {"wid", "n", "dtype", "segments", "times_ms": {config: ms}}.
+Each time is the median of a few back-to-back runs bracketed by events on the
+launch stream (fewer repetitions for long runs).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import statistics
+import sys
+import time
+from pathlib import Path
+from typing import Dict
+
+from .space import Config, Workload, c5_workloads, candidates
+
+
+class Runner:
+    """Device buffers for the largest workload and one handle per configuration."""
+
+    def __init__(self, max_n: int = 1 << 30, max_segments: int = 1 << 14):
+        import torch
+
+        import gpuprefixsums_amd as gps
+        self.torch, self.gps = torch, gps
+        self.max_n = max_n
+        dev = "cuda"
+        self.u32 = torch.empty(max_n, dtype=torch.int32, device=dev)
+        gps.fill_random_u32(self.u32, max_n, seed=6, bits=32)
+        g = torch.Generator(device=dev)
+        g.manual_seed(6)
+        self.f32 = torch.randn(max_n, dtype=torch.float32, device=dev, generator=g)
+        self.bf16 = self.f32.to(torch.bfloat16).view(torch.int16)
+        self.out = torch.empty(max_n, dtype=torch.int32, device=dev)
+        self.off = torch.empty(max_segments + 1, dtype=torch.int64, device=dev)
+        self.handles: Dict[Config, object] = {}
+        torch.cuda.synchronize()
+
+    def handle(self, c: Config):
+        gps = self.gps
+        h = self.handles.get(c)
+        if h is None:
+            algo = {"batched": gps.ALGO_CSDL, "loop_csdl": gps.ALGO_CSDL, "loop_rts": gps.ALGO_RTS,
+                    "loop_csdldf": gps.ALGO_CSDLDF, "loop_fp": gps.ALGO_CSDL}[c.kind]
+            h = gps.ScanHandle(algo, self.max_n, block_threads=c.block, vec_per_thread=c.vec,
+                               spin_limit=c.spin if c.kind == "loop_csdldf" else 0)
+            self.handles[c] = h
+        return h
+
+    def stage(self, w: Workload) -> None:
+        import numpy as np
+        self.off[: w.segments + 1].copy_(self.torch.from_numpy(w.offsets.view(np.int64)))
+
+    def run(self, w: Workload, c: Config) -> None:
+        h = self.handle(c)
+        if w.dtype == "u32":
+            if c.kind == "batched":
+                h.segmented(self.u32, self.out, self.off, w.segments, w.n, False)
+            else:
+                h.segmented_per_segment(self.u32, self.out, w.offsets, False)
+        else:
+            src = self.f32 if w.dtype == "f32" else self.bf16
+            h.fp_per_segment(src, self.out, w.offsets, bf16=w.dtype == "bf16", compensated=True)
+
+    def time(self, w: Workload, c: Config, best_so_far: float = float("inf")) -> float:
+        torch = self.torch
+        st = torch.cuda.current_stream()
+        self.run(w, c)  # warm-up
+        reps, out = 5, []
+        for r in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            self.run(w, c)
+            e1.record(st)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            out.append(ms)
+            # long or clearly losing runs need fewer repetitions
+            if ms > 20.0 or (r >= 1 and ms > 4.0 * best_so_far):
+                break
+        return statistics.median(out)
+
+    def time_all(self, w: Workload) -> Dict[str, float]:
+        self.stage(w)
+        res: Dict[str, float] = {}
+        best = float("inf")
+        for c in candidates(w.dtype):
+            t = self.time(w, c, best)
+            best = min(best, t)
+            res[c.name] = t
+        for h in self.handles.values():
+            if h.device_error():
+                raise RuntimeError(f"device error on workload {w.wid}")
+        return res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--start", type=int, default=0)
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=6)
+    args = ap.parse_args(argv)
+    ws = c5_workloads(args.seed)[args.start:args.start + args.count]
+    runner = Runner()
+    path = Path(args.out)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    t0 = time.time()
+    with path.open("a") as f:
+        for w in ws:
+            res = runner.time_all(w)
+            f.write(json.dumps({"wid": w.wid, "n": w.n, "dtype": w.dtype, "segments": w.segments,
+                                "times_ms": res}) + "\n")
+            f.flush()
+            print(f"[harness] workload {w.wid} n={w.n} {w.dtype} S={w.segments} best "
+                  f"{min(res, key=res.get)} {min(res.values()):.4f} ms ({time.time() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

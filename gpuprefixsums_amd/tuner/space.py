@@ -1,0 +1,131 @@
+"""Workloads, candidate configurations and feature encoding of the tuner.
+
+A workload is a batch of `segments` scans over `n` elements of one dtype
+(u32, f32 or bf16), stored back to back; one segment is a plain scan. A
+configuration fixes how it runs:
+
+  batched      the one-launch segmented kernel (u32)           tile shape
+  loop_csdl    one chained-scan launch per segment (u32)       tile shape
+  loop_rts     one reduce-then-scan launch per segment (u32)   tile shape
+  loop_csdldf  one decoupled-fallback launch per segment (u32) spin limit
+  loop_fp      one compensated FP scan launch per segment      tile shape
+
+Reference knobs these generalise: CSDL_WARPS / UINT4_PER_THREAD
+(GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:13-14) and
+MAX_SPIN_COUNT (GPUPrefixSumsCUDA/EmulatedDeadlocking.cuh:15).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List
+
+import numpy as np
+
+DTYPES = ("u32", "f32", "bf16")
+KINDS = ("batched", "loop_csdl", "loop_rts", "loop_csdldf", "loop_fp")
+
+
+@dataclass(frozen=True)
+class Config:
+    kind: str
+    block: int
+    vec: int
+    spin: int = 0
+
+    @property
+    def name(self) -> str:
+        s = f"{self.kind}:{self.block}x{self.vec}"
+        return s + (f":spin{self.spin}" if self.kind == "loop_csdldf" else "")
+
+    @staticmethod
+    def parse(name: str) -> "Config":
+        parts = name.split(":")
+        b, v = (int(t) for t in parts[1].split("x"))
+        spin = int(parts[2][4:]) if len(parts) > 2 else 0
+        return Config(parts[0], b, v, spin)
+
+
+U32_CONFIGS = (
+    [Config("batched", b, v) for b, v in ((256, 4), (512, 4), (1024, 2), (1024, 4))]
+    + [Config("loop_csdl", b, v) for b, v in ((1024, 8), (1024, 4), (512, 8), (256, 4))]
+    + [Config("loop_rts", b, v) for b, v in ((512, 2), (256, 4), (1024, 2))]
+    + [Config("loop_csdldf", 1024, 4, s) for s in (1, 4, 32)]
+)
+FP_CONFIGS = [Config("loop_fp", b, v) for b, v in ((1024, 8), (1024, 4), (256, 4))]
+
+
+def candidates(dtype: str) -> List[Config]:
+    return list(U32_CONFIGS) if dtype == "u32" else list(FP_CONFIGS)
+
+
+def default_config(dtype: str, segments: int) -> Config:
+    """What the library does without the tuner: the default CSDL shape for a
+    single scan, the default batched kernel for several segments, the default
+    FP shape for floating point."""
+    if dtype != "u32":
+        return Config("loop_fp", 1024, 8)
+    return Config("loop_csdl", 1024, 8) if segments == 1 else Config("batched", 1024, 4)
+
+
+@dataclass
+class Workload:
+    wid: int
+    n: int
+    dtype: str
+    segments: int
+    offsets: np.ndarray  # uint64[segments + 1], offsets[0] = 0, offsets[-1] = n
+
+    @property
+    def lengths(self) -> np.ndarray:
+        return np.diff(self.offsets.astype(np.int64))
+
+
+def lognormal_partition(n: int, segments: int, rng: np.random.Generator, sigma: float = 1.5) -> np.ndarray:
+    """Splits n into `segments` log-normal lengths (sigma 1.5) that sum to n
+    exactly (largest-remainder rounding; segments may be empty when
+    segments > n)."""
+    w = np.exp(sigma * rng.standard_normal(segments))
+    q = w / w.sum() * n
+    lengths = np.floor(q).astype(np.int64)
+    rem = n - int(lengths.sum())
+    if rem:
+        lengths[np.argsort(-(q - lengths), kind="stable")[:rem]] += 1
+    return np.concatenate([[0], np.cumsum(lengths)]).astype(np.uint64)
+
+
+def c5_workloads(seed: int = 6, count: int = 512) -> List[Workload]:
+    """The tuner sweep: n log-uniform in [2^10, 2^30], dtype uniform over
+    {u32, f32, bf16}, segment count log-uniform in [1, 2^14]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(2.0 ** rng.uniform(10, 30)))
+        dtype = DTYPES[int(rng.integers(3))]
+        segments = int(min(max(round(2.0 ** rng.uniform(0, 14)), 1), 1 << 14))
+        out.append(Workload(i, n, dtype, segments, lognormal_partition(n, segments, rng)))
+    return out
+
+
+def split(workloads: List[Workload], train: int = 384):
+    """Workloads are i.i.d., so the first `train` form the training set and the
+    rest are held out."""
+    return workloads[:train], workloads[train:]
+
+
+FEATURES = ["log2_n", "log2_segments", "log2_mean_len", "log2_max_len", "is_u32", "is_f32", "is_bf16"] + [
+    f"kind_{k}" for k in KINDS] + ["log2_block", "log2_vec", "log2_spin"]
+
+
+def workload_features(w: Workload) -> List[float]:
+    L = w.lengths
+    return [math.log2(w.n), math.log2(w.segments), math.log2(w.n / w.segments + 1.0), math.log2(float(L.max()) + 1.0),
+            float(w.dtype == "u32"), float(w.dtype == "f32"), float(w.dtype == "bf16")]
+
+
+def config_features(c: Config) -> List[float]:
+    return [float(c.kind == k) for k in KINDS] + [math.log2(c.block), math.log2(c.vec), math.log2(c.spin + 1.0)]
+
+
+def features(w: Workload, c: Config) -> List[float]:
+    return workload_features(w) + config_features(c)

@@ -63,6 +63,9 @@ typedef struct gps_tuning {
 #define GPS_SCHEDULE_DEFAULT 0
 #define GPS_SCHEDULE_TILE 1
 #define GPS_SCHEDULE_PERSISTENT 2
+/* CONTROL: persistent, with one wave per workgroup dedicated to tickets and
+ * lookback; tiles are (block_threads - 64) * vec_per_thread * 4 elements. */
+#define GPS_SCHEDULE_CONTROL 3
 
 typedef struct gps_handle gps_handle;
 
@@ -125,6 +128,36 @@ int gps_inclusive_scan_f32(gps_handle* h, const float* d_in, float* d_out, uint6
                            void* stream);
 int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, uint64_t n, int compensated,
                                 void* stream);
+
+/* ---- learned tuner: MLP cost model on the GPU (no reference counterpart) --- */
+
+/* A 3-layer perceptron (in_dim -> hidden -> hidden -> 1, ReLU) predicting
+ * log2 scan time from workload and configuration features, trained with MSE
+ * and Adam by HIP kernels. Parameters are one flat FP32 array:
+ * [W1 (hidden x in_dim), b1, W2 (hidden x hidden), b2, W3 (hidden), b3].
+ * Inputs, targets and outputs are host arrays (the data set is small). */
+typedef struct gps_mlp gps_mlp;
+uint64_t gps_mlp_param_count(uint32_t in_dim, uint32_t hidden);
+int gps_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t max_batch, gps_mlp** out);
+int gps_mlp_destroy(gps_mlp* m);
+/* Loads parameters and resets the optimiser state. */
+int gps_mlp_set_params(gps_mlp* m, const float* h_params);
+int gps_mlp_get_params(gps_mlp* m, float* h_params);
+/* y[batch] = model(x[batch x in_dim]). */
+int gps_mlp_forward(gps_mlp* m, const float* h_x, uint32_t batch, float* h_y);
+/* Mean squared error over the batch and its gradient w.r.t. every parameter
+ * (h_grad: param_count floats; either output may be NULL). */
+int gps_mlp_loss_grad(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch, float* h_loss,
+                      float* h_grad);
+/* Full-batch AdamW (beta1 0.9, beta2 0.999, eps 1e-8) for `epochs` steps;
+ * h_loss_curve (epochs floats, may be NULL) receives the loss before each step. */
+int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch, uint32_t epochs, float lr,
+                  float weight_decay, float* h_loss_curve);
+
+/* One FP scan launch per non-empty segment (host offsets; d_in is float or
+ * bf16 per `bf16`): the batched FP form, used by the tuner's workloads. */
+int gps_inclusive_scan_fp_per_segment(gps_handle* h, const void* d_in, float* d_out, const uint64_t* h_offsets,
+                                      uint64_t num_segments, int bf16, int compensated, void* stream);
 
 /* CSDLDF fallback counters accumulated since the last reset:
  * [0] status polls, [1] fallbacks (redundant tile reductions), [2] successful

@@ -1,0 +1,78 @@
+"""The tuner's MLP kernels on the GPU against the FP64 CPU oracle (oracle/mlp.py,
+itself pinned by finite differences), and the trained tuner's regret."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from gpuprefixsums_amd.tuner.model import CostModel, GpuMlp  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+
+
+def problem(D, H, B, seed):
+    rng = np.random.default_rng(seed)
+    p = omlp.init_params(D, H, seed) + rng.normal(0, 0.05, omlp.param_count(D, H))
+    x = rng.normal(size=(B, D))
+    t = rng.normal(size=B)
+    return p.astype(np.float32), x.astype(np.float32), t.astype(np.float32)
+
+
+@pytest.mark.parametrize("D,H,B", [(15, 64, 1), (15, 64, 300), (7, 9, 1000), (3, 128, 5000)])
+def test_forward_and_gradients_match_fp64_oracle(D, H, B):
+    p, x, t = problem(D, H, B, seed=D + H + B)
+    m = GpuMlp(D, H, B)
+    try:
+        assert m.count == omlp.param_count(D, H)
+        m.set_params(p)
+        assert np.array_equal(m.get_params(), p)
+        y = m.forward(x)
+        y64 = omlp.forward(p.astype(np.float64), x.astype(np.float64), D, H)
+        assert np.allclose(y, y64, rtol=1e-5, atol=1e-5 * (1 + np.abs(y64).max()))
+        loss, g = m.loss_grad(x, t)
+        L64, g64 = omlp.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), D, H)
+        assert abs(loss - L64) <= 1e-5 * max(1.0, L64)
+        scale = np.abs(g64).max()
+        err = np.abs(g - g64)
+        assert err.max() <= 2e-5 * scale + 1e-7, (err.max(), scale, int(err.argmax()))
+    finally:
+        m.close()
+
+
+def test_training_reduces_loss_and_fits():
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-2, 2, size=(4000, 6))
+    y = np.sin(X[:, 0]) + 0.5 * X[:, 1] ** 2 - X[:, 2] * X[:, 3]
+    model = CostModel(hidden=64).fit(X, y, epochs=1500, lr=3e-3)
+    assert model.curve[-1] < 0.05 * model.curve[0]
+    pred = model.predict(X)
+    assert np.sqrt(np.mean((pred - y) ** 2)) < 0.2
+
+
+def test_trained_tuner_regret_on_held_out_workloads():
+    """The shipped model's held-out regret against exhaustive search, and what
+    it gains over the library default (the method without the model)."""
+    from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
+    from gpuprefixsums_amd.tuner.evaluate import load_times, regrets
+    from gpuprefixsums_amd.tuner.space import c5_workloads, split
+    times_path = DATA / "c5_times_mi355x.jsonl"
+    if not times_path.exists() or not DEFAULT_MODEL.exists():
+        pytest.skip("tuner data not generated yet")
+    times = load_times(times_path)
+    ws = [w for w in c5_workloads() if w.wid in times]
+    _, held = split(ws)
+    r_mlp, r_def, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times)
+    print(json.dumps({"mlp": r_mlp, "default": r_def}))
+    assert r_mlp["mean"] <= 0.05
+    assert r_mlp["mean"] < r_def["mean"]

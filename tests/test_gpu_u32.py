@@ -23,6 +23,7 @@ from tests.gpu_util import (empty_u32, first_mismatch, from_dev_u32, sentinel_u3
 ALGOS = {
     "csdl": dict(algo=gps.ALGO_CSDL),  # persistent pipelined schedule (default)
     "csdl_tile": dict(algo=gps.ALGO_CSDL, schedule=gps.SCHEDULE_TILE),  # one workgroup per tile
+    "csdl_control": dict(algo=gps.ALGO_CSDL, schedule=gps.SCHEDULE_CONTROL),  # persistent + control wave
     "rts": dict(algo=gps.ALGO_RTS),
     "csdldf": dict(algo=gps.ALGO_CSDLDF),
     "csdldf_stall7": dict(algo=gps.ALGO_CSDLDF, stall_period=7),
@@ -109,11 +110,11 @@ def test_reference_ones_sweep_one_to_two_tiles(name, inclusive):
 
 
 @pytest.mark.parametrize("block,vec", [(b, v) for b in (256, 512, 1024) for v in (2, 4, 8)])
-@pytest.mark.parametrize("name", ["csdl", "csdl_tile", "rts", "csdldf_stall7"])
+@pytest.mark.parametrize("name", ["csdl", "csdl_tile", "csdl_control", "rts", "csdldf_stall7"])
 def test_every_tile_shape(block, vec, name):
     with gps.ScanHandle(max_n=1 << 20, block_threads=block, vec_per_thread=vec, **ALGOS[name]) as h:
         T = h.tile
-        assert T == block * vec * 4
+        assert T == (block - 64 if name == "csdl_control" else block) * vec * 4
         for n in (1, T - 1, T, T + 1, 3 * T + 5, (1 << 20) - 3):
             x = oracle.gen_u32(n * 3 + block + vec, n, 32)
             check(h, x, True, f"{block}x{vec}")

@@ -218,3 +218,31 @@ def test_bf16_generator_magnitudes():
     assert abs(lg.mean()) < 0.1  # U(-4, 4) in the exponent
     s = np.sign(oracle.bf16_bits_to_f32(bits))
     assert abs(s.mean()) < 0.05
+
+
+# ---- MLP cost-model oracle (FP64) -----------------------------------------
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp
+    D, H, B = 7, 9, 13
+    rng = np.random.default_rng(0)
+    p = mlp.init_params(D, H, 1) + rng.normal(0, 0.05, mlp.param_count(D, H))
+    x = rng.normal(size=(B, D))
+    t = rng.normal(size=B)
+    L, g = mlp.loss_grad(p, x, t, D, H)
+    assert abs(L - mlp.loss(p, x, t, D, H)) < 1e-12
+    h = 1e-6
+    for i in range(p.size):
+        e = np.zeros_like(p)
+        e[i] = h
+        fd = (mlp.loss(p + e, x, t, D, H) - mlp.loss(p - e, x, t, D, H)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_mlp_oracle_layout():
+    from oracle import mlp
+    D, H = 3, 4
+    assert mlp.param_count(D, H) == 4 * 3 + 4 + 16 + 4 + 4 + 1
+    p = np.arange(mlp.param_count(D, H), dtype=np.float64)
+    W1, b1, W2, b2, W3, b3 = mlp.unpack(p, D, H)
+    assert W1[1, 2] == 5 and b1[0] == 12 and W2[0, 0] == 16 and b2[3] == 35 and W3[0] == 36 and b3 == 40
