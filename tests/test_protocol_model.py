@@ -1,0 +1,178 @@
+"""CPU model of the tile-status protocol (lookback, decoupled fallback, emulated
+deadlock, epochs), run under random interleavings. No GPU.
+
+The model mirrors the device code step by step (gpuprefixsums_amd/csrc/
+device_common.hpp: status words, tickets, window lookback;
+scan_u32.hip: csdldf_u32): every shared-memory access is one step, and a
+seeded scheduler picks which resident tile steps next. Tiles become resident
+in ticket order, at most `resident` at a time. Several launches reuse one
+status array, so stale words from earlier epochs are present. The checks:
+each tile's exclusive prefix equals the true prefix mod 2^32, and across the
+runs every fallback branch is taken (timeout, CAS win, CAS lost to an
+aggregate, CAS lost to an inclusive word, fallback onto tile 0, stalled tile
+released by a successor).
+"""
+import random
+from collections import Counter
+
+import pytest
+
+MASK32 = 0xFFFFFFFF
+AGG, INCL = 1, 2
+EPOCH_MASK = (1 << 30) - 1
+
+
+def make_status(value, flag, epoch):
+    return (value & MASK32) | (flag << 32) | ((epoch & EPOCH_MASK) << 34)
+
+
+def flag_of(w, epoch):
+    return (w >> 32) & 3 if (w >> 34) == (epoch & EPOCH_MASK) else 0
+
+
+class Memory:
+    def __init__(self, tiles):
+        self.status = [0] * tiles
+        self.ticket = 1 << 32
+
+    def cas(self, i, expected, desired):
+        found = self.status[i]
+        if found == expected:
+            self.status[i] = desired
+        return found
+
+
+def tile_program(mem, t, agg, epoch, num_tiles, period, spin_limit, stats, result):
+    """Generator: one tile of the decoupled-fallback kernel (one yield per access)."""
+    # stall emulation: tiles t % period == 0 that have a successor wait until
+    # their own word has been inserted by a successor
+    if period and t % period == 0 and t + 1 < num_tiles:
+        while True:
+            w = mem.status[t]
+            yield
+            if flag_of(w, epoch):
+                stats["stall_released"] += 1
+                break
+    mem.status[t] = make_status(agg[t], AGG if t else INCL, epoch)  # exchange
+    yield
+    if t == 0:
+        result[t] = 0
+        return
+    prefix, top, spins = 0, t - 1, 0
+    while True:
+        # window read: lane l reads tile top - l (one step per word)
+        words = []
+        for l in range(min(64, top + 1)):
+            words.append(mem.status[top - l])
+            yield
+        flags = [flag_of(w, epoch) for w in words]
+        run = 0
+        while run < len(flags) and flags[run]:
+            run += 1
+        first_incl = next((i for i, f in enumerate(flags) if f == INCL), len(flags))
+        if first_incl < run:
+            prefix += sum(w & MASK32 for w in words[:first_incl + 1])
+            break
+        if run:
+            prefix += sum(w & MASK32 for w in words[:run])
+            top -= run
+            spins = 0
+            continue
+        spins += 1
+        if spins < spin_limit:
+            continue
+        # fallback onto tile `top`: reduce its input, CAS over the stale word
+        stats["fallback"] += 1
+        j, stale = top, words[0]
+        yield  # the redundant reduction of tile j's input
+        mine = make_status(agg[j], AGG if j else INCL, epoch)
+        found = mem.cas(j, stale, mine)
+        yield
+        if found == stale:
+            stats["cas_win"] += 1
+            found, fl = mine, (AGG if j else INCL)
+        else:
+            fl = flag_of(found, epoch)
+            assert fl, "a lost CAS must return a current-epoch word"
+            stats["cas_lost_incl" if fl == INCL else "cas_lost_agg"] += 1
+        if j == 0:
+            stats["fallback_tile0"] += 1
+        prefix += found & MASK32
+        spins = 0
+        if fl == INCL:
+            break
+        top = j - 1
+    mem.status[t] = make_status(prefix + agg[t], INCL, epoch)
+    yield
+    result[t] = prefix & MASK32
+
+
+def run_launch(mem, agg, epoch, period, spin_limit, resident, rng, stats):
+    num_tiles = len(agg)
+    result = [None] * num_tiles
+    progs, next_tile, active = {}, 0, []
+    steps = 0
+    while next_tile < num_tiles or active:
+        while len(active) < resident and next_tile < num_tiles:  # dispatch in ticket order
+            progs[next_tile] = tile_program(mem, next_tile, agg, epoch, num_tiles, period, spin_limit, stats, result)
+            active.append(next_tile)
+            next_tile += 1
+        t = rng.choice(active)
+        try:
+            next(progs[t])
+        except StopIteration:
+            active.remove(t)
+        steps += 1
+        assert steps < 5_000_000, "protocol did not terminate"
+    return result
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_protocol_under_random_interleavings(seed):
+    rng = random.Random(seed)
+    num_tiles = rng.choice([1, 2, 9, 40, 130])
+    mem = Memory(200)
+    stats = Counter()
+    for launch in range(4):  # reuse the status array: stale words from older epochs
+        epoch = launch + 1
+        agg = [rng.getrandbits(32) for _ in range(num_tiles)]
+        period = rng.choice([0, 2, 7, 8])
+        res = run_launch(mem, agg, epoch, period, spin_limit=rng.choice([1, 2, 4]),
+                         resident=rng.choice([2, 5, 17]), rng=rng, stats=stats)
+        acc = 0
+        for t in range(num_tiles):
+            assert res[t] == acc, (seed, launch, t)
+            acc = (acc + agg[t]) & MASK32
+        # after the launch every word is inclusive for this epoch
+        assert all(flag_of(mem.status[t], epoch) == INCL for t in range(num_tiles))
+
+
+def test_every_fallback_branch_is_exercised():
+    stats = Counter()
+    for seed in range(40):
+        rng = random.Random(1000 + seed)
+        mem = Memory(64)
+        agg = [rng.getrandbits(32) for _ in range(48)]
+        res = run_launch(mem, agg, 1 + seed % 3, period=rng.choice([2, 7, 8]), spin_limit=1,
+                         resident=rng.choice([3, 9]), rng=rng, stats=stats)
+        acc = 0
+        for t, r in enumerate(res):
+            assert r == acc
+            acc = (acc + agg[t]) & MASK32
+    for branch in ("fallback", "cas_win", "cas_lost_agg", "cas_lost_incl", "fallback_tile0", "stall_released"):
+        assert stats[branch] > 0, (branch, dict(stats))
+
+
+def test_stale_epoch_words_are_never_consumed():
+    # A status array full of inclusive words from an older epoch must look
+    # "not ready" to the new launch: results stay exact.
+    rng = random.Random(7)
+    mem = Memory(50)
+    mem.status = [make_status(rng.getrandbits(32), INCL, 5) for _ in range(50)]
+    agg = [rng.getrandbits(32) for _ in range(50)]
+    stats = Counter()
+    res = run_launch(mem, agg, 6, period=0, spin_limit=3, resident=4, rng=rng, stats=stats)
+    acc = 0
+    for t, r in enumerate(res):
+        assert r == acc
+        acc = (acc + agg[t]) & MASK32
