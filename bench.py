@@ -11,7 +11,10 @@ progress goes to stderr. The CPU baseline is the serial host scan (the
 reference's validator loop restated in C) timed on this machine's host.
 
 Extra legs (reported under "extras"): csdl_exclusive, rts, csdldf (stall
-periods 0/7/8), copy (the read-n + write-n ceiling), n2p20 (latency at 2^20).
+periods 0/7/8), copy (the read-n + write-n ceiling), n2p20 (latency at 2^20),
+segmented (4096 log-normal segments: one graph-launched pass vs one launch
+per segment), fp (n = 2^27 FP32/BF16 with and without compensated carries,
+errors vs FP64), tuner (held-out regret of the learned cost model).
 """
 from __future__ import annotations
 
@@ -29,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
-ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "n2p20", "segmented", "fp"]
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "n2p20", "segmented", "fp", "tuner"]
 
 
 def log(msg: str) -> None:
@@ -198,6 +201,25 @@ class Legs:
             if h.device_error():
                 raise RuntimeError("device error in fp leg")
         out["fp_n2p27"] = res
+
+    def tuner(self, out):
+        """Learned tuner: regret of the MLP cost model's choices (inference on
+        the GPU with the library's kernels) on the 128 held-out workloads of the
+        512-workload sweep, against the exhaustively measured best, and the
+        regret of the library default (no model) on the same workloads."""
+        from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
+        from gpuprefixsums_amd.tuner.evaluate import load_times, regrets
+        from gpuprefixsums_amd.tuner.model import CostModel
+        from gpuprefixsums_amd.tuner.space import c5_workloads, split
+        times_path = DATA / "c5_times_mi355x.jsonl"
+        if not times_path.exists() or not DEFAULT_MODEL.exists():
+            out["tuner"] = None
+            return
+        times = load_times(times_path)
+        _, held = split([w for w in c5_workloads() if w.wid in times])
+        r_mlp, r_def, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times)
+        out["tuner"] = {"held_out": len(held), "regret_mlp": r_mlp, "regret_default": r_def,
+                        "source": "tuner/data/c5_times_mi355x.jsonl (exhaustive harness timings on MI355X)"}
 
     def n2p20(self, out):
         gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
