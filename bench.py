@@ -11,7 +11,8 @@ progress goes to stderr. The CPU baseline is the serial host scan (the
 reference's validator loop restated in C) timed on this machine's host.
 
 Extra legs (reported under "extras"): csdl_exclusive, rts, csdldf (stall
-periods 0/7/8), copy (the read-n + write-n ceiling), n2p20 (latency at 2^20),
+periods 0/7/8), copy (the read-n + write-n ceiling), vendor (rocPRIM's device
+scan, the reference's CUB comparator on ROCm), n2p20 (latency at 2^20),
 segmented (4096 log-normal segments: one graph-launched pass vs one launch
 per segment), fp (n = 2^27 FP32/BF16 with and without compensated carries,
 errors vs FP64), tuner (held-out regret of the learned cost model).
@@ -32,7 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
-ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "n2p20", "segmented", "fp", "tuner"]
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "fp", "tuner"]
 
 
 def log(msg: str) -> None:
@@ -116,6 +117,21 @@ class Legs:
         e1.record(st)
         torch.cuda.synchronize()
         out["copy_ceiling_gbps"] = round(gbps(self.n, e0.elapsed_time(e1) / 20), 1)
+
+    def vendor(self, out):
+        """rocPRIM's device scan (the reference's CUB comparator on ROCm) on the
+        same input, separate buffers, inclusive, checked before it is reported."""
+        gps, torch = self.gps, self.torch
+        tb = gps.vendor_scan_temp_bytes(self.n, True)
+        temp = torch.empty(max(tb, 4), dtype=torch.uint8, device="cuda")
+        for _ in range(3):
+            gps.vendor_scan_u32(self.d_in, self.d_out, self.n, temp, tb, True)
+        ms = gps.vendor_time_scan_u32(self.d_in, self.d_out, self.n, temp, tb, True, iters=20)
+        torch.cuda.synchronize()
+        errors = gps.validate_scan_u32(self.d_in, self.d_out, self.n, True)
+        mean = sum(ms) / len(ms)
+        out["rocprim_inclusive"] = {"gbps": round(gbps(self.n, mean), 1), "kernel_ms_mean": round(mean, 5),
+                                    "validation_errors": errors, "temp_bytes": tb}
 
     def segmented(self, out):
         """4096 segments, log-normal lengths (median 4096, sigma 1.5, clipped to

@@ -24,7 +24,8 @@ __all__ = [
     "ALGO_CSDL", "ALGO_RTS", "ALGO_CSDLDF", "GpsError", "ScanHandle",
     "ChainedScanDecoupledLookbackDispatcher", "ReduceThenScanDispatcher",
     "EmulatedDeadlockingDispatcher", "fill_u32", "fill_random_u32", "validate_scan_u32",
-    "validate_ones_u32", "load",
+    "validate_ones_u32", "load", "vendor_scan_temp_bytes", "vendor_scan_u32", "vendor_time_scan_u32",
+    "CubDispatcher",
 ]
 
 
@@ -231,6 +232,30 @@ def validate_ones_u32(d_scan, n: int, inclusive: bool = True, stream=None) -> in
     return e.value
 
 
+# ---- vendor comparator (rocPRIM), CubDispatcher.cuh:32-76 ------------------
+
+def vendor_scan_temp_bytes(n: int, inclusive: bool = True) -> int:
+    b = ctypes.c_uint64()
+    check(load().gps_vendor_scan_temp_bytes(n, int(inclusive), ctypes.byref(b)))
+    return b.value
+
+
+def vendor_scan_u32(d_in, d_out, n: int, d_temp, temp_bytes: int, inclusive: bool = True, stream=None) -> None:
+    """rocPRIM's device scan of n uint32 (d_in == d_out allowed); d_temp holds
+    at least vendor_scan_temp_bytes(n, inclusive) bytes."""
+    check(load().gps_vendor_scan_u32(_ptr(d_in), _ptr(d_out), n, int(inclusive), _ptr(d_temp), temp_bytes,
+                                     _stream(stream)))
+
+
+def vendor_time_scan_u32(d_in, d_out, n: int, d_temp, temp_bytes: int, inclusive: bool = True, iters: int = 10,
+                         stream=None) -> list:
+    """Per-launch milliseconds of `iters` back-to-back rocPRIM scans."""
+    ms = (ctypes.c_float * iters)()
+    check(load().gps_vendor_time_scan_u32(_ptr(d_in), _ptr(d_out), n, int(inclusive), _ptr(d_temp), temp_bytes,
+                                          _stream(stream), iters, ms))
+    return list(ms)
+
+
 # ---- reference dispatcher classes (self-tests run in the C++ classes) -----
 
 class _Dispatcher:
@@ -262,3 +287,17 @@ class ReduceThenScanDispatcher(_Dispatcher):
 
 class EmulatedDeadlockingDispatcher(_Dispatcher):
     _algo = ALGO_CSDLDF
+
+
+class CubDispatcher:
+    """The reference's vendor comparator (CubDispatcher.cuh:21-76) on rocPRIM:
+    BatchTimingCubChainedScan times the vendor inclusive scan in place over
+    all-ones input, first iteration discarded, and returns keys/s."""
+
+    def __init__(self, max_size: int):
+        self.max_size = max_size
+
+    def batch_timing_cub_chained_scan(self, size: int, batch_count: int) -> float:
+        keys = ctypes.c_double()
+        check(load().gps_vendor_batch_timing(self.max_size, size, batch_count, ctypes.byref(keys)))
+        return keys.value

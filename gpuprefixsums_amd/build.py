@@ -31,6 +31,7 @@ LIB_SOURCES = [
     "segmented_u32.hip",
     "scan_fp.hip",
     "mlp.hip",
+    "vendor_rocprim.hip",
     "capi.cpp",
     "capi_ext.cpp",
     "dispatchers.cpp",

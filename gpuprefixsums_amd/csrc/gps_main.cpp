@@ -4,9 +4,8 @@
 //   gps_main [log2_max_size] [batch]     defaults 28 and 100
 //
 // For each dispatcher: TestAllExclusive / TestAllInclusive, then
-// BatchTimingInclusive(1 << log2_max_size, batch). The reference's CUB timing
-// leg (GPUPrefixSumsCUDA/CubDispatcher.cuh:32-76) has no counterpart: the
-// library calls no vendor scan.
+// BatchTimingInclusive(1 << log2_max_size, batch); then the vendor comparator
+// (GPUPrefixSumsCUDA/CubDispatcher.cuh:32-76), which on ROCm times rocPRIM.
 #include <cstdio>
 #include <cstdlib>
 
@@ -45,6 +44,10 @@ int main(int argc, char** argv) {
             ed.TestAllExclusive();
             failures += ed.TestsPassed() != ed.TestsRun();
             ed.BatchTimingInclusive(size, batch);
+        }
+        {
+            gps::CubDispatcher cub(size);
+            cub.BatchTimingCubChainedScan(size, batch);
         }
     } catch (const gps::Error& e) {
         fprintf(stderr, "gps error %d: %s\n", e.code, e.what());

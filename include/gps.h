@@ -197,12 +197,29 @@ int gps_validate_ones_u32(const uint32_t* d_scan, uint64_t n, int inclusive, voi
 int gps_validate_scan_u32(const uint32_t* d_in, const uint32_t* d_scan, uint64_t n, int inclusive,
                           void* stream, uint64_t* errors);
 
+/* ---- vendor comparator: rocPRIM's device scan ----------------------------
+ * Replaces the reference's CUB comparator (CubDispatcher.cuh:32-76:
+ * cub::DeviceScan::InclusiveSum with a two-phase temporary-storage query).
+ * On ROCm the vendor scan is rocprim::inclusive_scan / exclusive_scan.
+ * Caller buffers (d_in == d_out allowed), caller temporary storage of at
+ * least gps_vendor_scan_temp_bytes(n, inclusive) bytes, no handle. */
+int gps_vendor_scan_temp_bytes(uint64_t n, int inclusive, uint64_t* bytes);
+int gps_vendor_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* d_temp,
+                        uint64_t temp_bytes, void* stream);
+/* `iters` back-to-back vendor scans, per-launch hipEvent times into ms[iters]
+ * (the timing loop of CubDispatcher.cuh:52-68). */
+int gps_vendor_time_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* d_temp,
+                             uint64_t temp_bytes, void* stream, uint32_t iters, float* ms);
+
 /* ---- reference dispatcher self-tests (C++ classes in gps_dispatchers.hpp) - */
 
 /* Runs <Dispatcher>::TestAllInclusive / TestAllExclusive for algo with a
  * dispatcher of max_size elements; writes tests passed and tests run.
  * (ChainedScanDecoupledLookbackDispatcher.cuh:38-114) */
 int gps_dispatcher_test_all(int algo, uint32_t max_size, int inclusive, uint32_t* passed, uint32_t* total);
+/* Runs CubDispatcher(max_size).BatchTimingCubChainedScan(size, batch_count)
+ * on the rocPRIM comparator and writes its keys/s (CubDispatcher.cuh:32-76). */
+int gps_vendor_batch_timing(uint32_t max_size, uint32_t size, uint32_t batch_count, double* keys_per_sec);
 
 #ifdef __cplusplus
 }
