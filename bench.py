@@ -224,7 +224,7 @@ class Legs:
         512-workload sweep, against the exhaustively measured best, and the
         regret of the library default (no model) on the same workloads."""
         from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
-        from gpuprefixsums_amd.tuner.evaluate import load_times, regrets
+        from gpuprefixsums_amd.tuner.evaluate import best_static, load_times, regrets
         from gpuprefixsums_amd.tuner.model import CostModel
         from gpuprefixsums_amd.tuner.space import c5_workloads, split
         times_path = DATA / "c5_times_mi355x.jsonl"
@@ -232,9 +232,11 @@ class Legs:
             out["tuner"] = None
             return
         times = load_times(times_path)
-        _, held = split([w for w in c5_workloads() if w.wid in times])
-        r_mlp, r_def, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times)
+        train, held = split([w for w in c5_workloads() if w.wid in times])
+        static = best_static(train, times)
+        r_mlp, r_def, r_stat, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times, static)
         out["tuner"] = {"held_out": len(held), "regret_mlp": r_mlp, "regret_default": r_def,
+                        "regret_best_static": r_stat, "best_static": static,
                         "source": "tuner/data/c5_times_mi355x.jsonl (exhaustive harness timings on MI355X)"}
 
     def n2p20(self, out):

@@ -19,6 +19,13 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// Wave index inside the workgroup, in a scalar register: branches on it are
+// uniform to the compiler, so role-split code (one wave polls, the others
+// stream) is not structured as a divergent region.
+__device__ __forceinline__ uint32_t wave_index() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
+
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ uint32_t readlane(uint32_t x, int lane) {
@@ -173,8 +180,11 @@ struct Lookback {
         empty_polls = 0;
     }
     __device__ __forceinline__ void issue(const uint64_t* status, uint32_t epoch) {
+        // Branch-free: every lane loads (lanes below tile 0 reload word 0)
+        // and selects, so no divergent region precedes consume()'s ballots.
         const int64_t idx = top - (int64_t)lane_id();
-        w = idx >= 0 ? atomic_load_agent(status + idx) : make_status(0u, kFlagIncl, epoch);
+        const uint64_t v = atomic_load_agent(status + (idx >= 0 ? idx : 0));
+        w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
     }
     // Folds the issued window in; true once `prefix` is complete. Sets
     // `gave_up` (and returns true) when the watchdog fires.

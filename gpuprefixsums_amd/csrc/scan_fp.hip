@@ -93,12 +93,14 @@ __device__ __forceinline__ ff lookback_ff(const uint64_t* status, uint32_t tile,
     int64_t top = (int64_t)tile - 1;
     uint32_t empty_polls = 0;
     while (true) {
+        // Branch-free window read: lanes below tile 0 reload pair 0 and take
+        // the sentinel, so no divergent region precedes the ballots.
         const int64_t idx = top - (int64_t)lane;
-        uint64_t ws = make_status(0u, kFlagIncl, epoch), wc = ws;
-        if (idx >= 0) {
-            ws = atomic_load_agent(status + 2 * idx);
-            wc = atomic_load_agent(status + 2 * idx + 1);
-        }
+        const int64_t at = idx >= 0 ? idx : 0;
+        const uint64_t ls = atomic_load_agent(status + 2 * at);
+        const uint64_t lc = atomic_load_agent(status + 2 * at + 1);
+        const uint64_t sentinel = make_status(0u, kFlagIncl, epoch);
+        const uint64_t ws = idx >= 0 ? ls : sentinel, wc = idx >= 0 ? lc : sentinel;
         const uint32_t fs = status_flag(ws, epoch), fc = status_flag(wc, epoch);
         const uint32_t f = fs == fc ? fs : 0u;
         const uint64_t ready = ballot(f != 0u);

@@ -252,7 +252,7 @@ csdl_cw_u32(ScanParams p) {
     __shared__ uint32_t s_ticket[2];
     __shared__ uint32_t s_prefix[2];
     __shared__ uint32_t s_ticket_seq, s_prefix_seq;
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_index();
     const uint32_t lane = lane_id();
     const uint32_t total_tickets = p.num_tiles + gridDim.x;
     if (threadIdx.x < DW) s_arrived[threadIdx.x] = 0;
@@ -416,7 +416,9 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                 uint32_t done = 0;
                 while (true) {
                     const int64_t idx = top - (int64_t)lane;
-                    const uint64_t w = idx >= 0 ? atomic_load_agent(p.status + idx) : make_status(0u, kFlagIncl, epoch);
+                    // branch-free window read (no divergent region before the ballots)
+                    const uint64_t v = atomic_load_agent(p.status + (idx >= 0 ? idx : 0));
+                    const uint64_t w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
                     const uint32_t fl = status_flag(w, epoch);
                     const uint64_t ready = ballot(fl != 0u);
                     const uint64_t incl = ballot(fl == (uint32_t)kFlagIncl);

@@ -64,15 +64,16 @@ def test_trained_tuner_regret_on_held_out_workloads():
     """The shipped model's held-out regret against exhaustive search, and what
     it gains over the library default (the method without the model)."""
     from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
-    from gpuprefixsums_amd.tuner.evaluate import load_times, regrets
+    from gpuprefixsums_amd.tuner.evaluate import best_static, load_times, regrets
     from gpuprefixsums_amd.tuner.space import c5_workloads, split
     times_path = DATA / "c5_times_mi355x.jsonl"
     if not times_path.exists() or not DEFAULT_MODEL.exists():
         pytest.skip("tuner data not generated yet")
     times = load_times(times_path)
     ws = [w for w in c5_workloads() if w.wid in times]
-    _, held = split(ws)
-    r_mlp, r_def, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times)
-    print(json.dumps({"mlp": r_mlp, "default": r_def}))
+    train, held = split(ws)
+    r_mlp, r_def, r_stat, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times, best_static(train, times))
+    print(json.dumps({"mlp": r_mlp, "default": r_def, "best_static": r_stat}))
     assert r_mlp["mean"] <= 0.05
     assert r_mlp["mean"] < r_def["mean"]
+    assert r_mlp["mean"] < r_stat["mean"]

@@ -1,0 +1,98 @@
+"""Host-side logic of the tuner (no GPU): the workload sweep, candidate
+space, feature encoding, and the regret arithmetic used to judge the model.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gpuprefixsums_amd.tuner.evaluate import best_static, regrets
+from gpuprefixsums_amd.tuner.space import (FEATURES, Config, Workload, c5_workloads, candidates, default_config,
+                                           features, lognormal_partition, split)
+
+
+def test_sweep_shape_and_split():
+    ws = c5_workloads()
+    assert len(ws) == 512
+    train, held = split(ws)
+    assert len(train) == 384 and len(held) == 128
+    assert [w.wid for w in ws] == list(range(512))
+    for w in ws:
+        assert 2 ** 10 - 1 <= w.n <= 2 ** 30 + 1
+        assert 1 <= w.segments <= 2 ** 14
+        assert w.dtype in ("u32", "f32", "bf16")
+        assert w.offsets[0] == 0 and int(w.offsets[-1]) == w.n
+        assert np.all(np.diff(w.offsets.astype(np.int64)) >= 0)
+    # seeded: the same sweep every time
+    again = c5_workloads()
+    assert all(a.n == b.n and a.dtype == b.dtype and np.array_equal(a.offsets, b.offsets) for a, b in zip(ws, again))
+
+
+def test_lognormal_partition_sums_exactly():
+    rng = np.random.default_rng(1)
+    for n, s in [(1, 1), (5, 9), (1000, 7), (12345, 4096)]:
+        off = lognormal_partition(n, s, rng)
+        assert off.size == s + 1 and int(off[-1]) == n
+
+
+def test_config_names_round_trip_and_candidates():
+    for dt in ("u32", "f32", "bf16"):
+        cs = candidates(dt)
+        assert len({c.name for c in cs}) == len(cs)
+        for c in cs:
+            assert Config.parse(c.name) == c
+        assert default_config(dt, 1).name in {c.name for c in cs}
+        assert default_config(dt, 100).name in {c.name for c in cs}
+    assert all(c.kind == "loop_fp" for c in candidates("f32"))
+
+
+def test_features_dimension_and_values():
+    w = Workload(0, 1 << 20, "bf16", 4, np.array([0, 1, 2, 3, 1 << 20], dtype=np.uint64))
+    f = features(w, Config("loop_csdldf", 1024, 4, 32))
+    assert len(f) == len(FEATURES) == 15
+    d = dict(zip(FEATURES, f))
+    assert d["log2_n"] == 20 and d["log2_segments"] == 2
+    assert d["is_bf16"] == 1 and d["is_u32"] == 0
+    assert d["kind_loop_csdldf"] == 1 and d["log2_block"] == 10 and d["log2_spin"] == math.log2(33)
+
+
+class _StubModel:
+    """Predicts a fixed log-time per configuration name."""
+
+    def __init__(self, table):
+        self.table = table
+        self.names = []
+
+    def predict(self, X):
+        # features() puts the kind one-hot and the shape last; recover the name
+        out = []
+        for row in np.asarray(X):
+            kind = [k for k, v in zip(("batched", "loop_csdl", "loop_rts", "loop_csdldf", "loop_fp"), row[7:12])
+                    if v == 1][0]
+            c = Config(kind, int(round(2 ** row[12])), int(round(2 ** row[13])),
+                       int(round(2 ** row[14] - 1)) if kind == "loop_csdldf" else 0)
+            out.append(self.table.get(c.name, 100.0))
+        return np.array(out)
+
+
+def test_regret_arithmetic_against_hand_computed_values():
+    ws = [Workload(i, 4096, "f32", 2, np.array([0, 2048, 4096], dtype=np.uint64)) for i in range(4)]
+    names = [c.name for c in candidates("f32")]  # 1024x8, 1024x4, 256x4
+    times = {
+        0: {names[0]: 2.0, names[1]: 1.0, names[2]: 4.0},
+        1: {names[0]: 3.0, names[1]: 1.5, names[2]: 1.0},
+        2: {names[0]: 1.0, names[1]: 2.0, names[2]: 2.0},
+        3: {names[0]: 5.0, names[1]: 1.0, names[2]: 1.0},
+    }
+    # best static on workloads 0..1: mean regrets 1024x8: (1 + 2)/2 = 1.5; 1024x4: (0 + .5)/2 = .25; 256x4: (3+0)/2
+    static = best_static(ws[:2], times)
+    assert static == {"f32": names[1]}
+    model = _StubModel({names[2]: 0.0, names[1]: 1.0, names[0]: 2.0})  # always picks 256x4
+    r_mlp, r_def, r_stat, rows = regrets(model, ws[2:], times, static)
+    # held out = workloads 2, 3. MLP picks 256x4: regrets 2/1-1 = 1, 1/1-1 = 0
+    assert r_mlp["mean"] == pytest.approx(0.5) and r_mlp["max"] == pytest.approx(1.0)
+    # default 1024x8: 0 and 4
+    assert r_def["mean"] == pytest.approx(2.0)
+    # static 1024x4: 1 and 0
+    assert r_stat["mean"] == pytest.approx(0.5)
+    assert [r["mlp"] for r in rows] == [names[2], names[2]]
