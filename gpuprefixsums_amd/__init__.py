@@ -232,6 +232,15 @@ def validate_ones_u32(d_scan, n: int, inclusive: bool = True, stream=None) -> in
     return e.value
 
 
+WAVE_INCLUSIVE_SCAN, WAVE_SUM, WAVE_EXCLUSIVE_SCAN = 0, 1, 2
+
+
+def wave_primitive_u32(d_in, d_out, n: int, op: int, stream=None) -> None:
+    """Runs one wave64 primitive (Utils.cuh:22-71 counterparts) on every run
+    of 64 consecutive elements: WAVE_INCLUSIVE_SCAN, WAVE_SUM, WAVE_EXCLUSIVE_SCAN."""
+    check(load().gps_wave_primitive_u32(_ptr(d_in), _ptr(d_out), n, op, _stream(stream)))
+
+
 # ---- vendor comparator (rocPRIM), CubDispatcher.cuh:32-76 ------------------
 
 def vendor_scan_temp_bytes(n: int, inclusive: bool = True) -> int:

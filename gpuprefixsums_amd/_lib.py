@@ -88,6 +88,7 @@ SIGNATURES = {
     "gps_validate_ones_u32": (_int, [_vp, _u64, _int, _vp, ctypes.POINTER(_u64)]),
     "gps_validate_scan_u32": (_int, [_vp, _vp, _u64, _int, _vp, ctypes.POINTER(_u64)]),
     "gps_dispatcher_test_all": (_int, [_int, _u32, _int, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "gps_wave_primitive_u32": (_int, [_vp, _vp, _u64, _int, _vp]),
     "gps_vendor_scan_temp_bytes": (_int, [_u64, _int, ctypes.POINTER(_u64)]),
     "gps_vendor_scan_u32": (_int, [_vp, _vp, _u64, _int, _vp, _u64, _vp]),
     "gps_vendor_time_scan_u32": (_int, [_vp, _vp, _u64, _int, _vp, _u64, _vp, _u32, _fp]),

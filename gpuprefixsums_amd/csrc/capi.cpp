@@ -291,6 +291,15 @@ int gps_fill_u32(uint32_t* d_ptr, uint64_t n, uint32_t value, void* stream) {
     return GPS_OK;
 }
 
+int gps_wave_primitive_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int op, void* stream) {
+    if (n && (!d_in || !d_out)) return fail(GPS_ERR_INVALID, "null pointer");
+    if (op < GPS_WAVE_INCLUSIVE_SCAN || op > GPS_WAVE_EXCLUSIVE_SCAN) return fail(GPS_ERR_INVALID, "unknown op %d", op);
+    if ((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) & 3u)
+        return fail(GPS_ERR_INVALID, "uint32 buffers must be 4-byte aligned");
+    HIP_TRY(gps::launch_wave_unit_u32(d_in, d_out, n, (uint32_t)op, static_cast<hipStream_t>(stream)));
+    return GPS_OK;
+}
+
 int gps_copy_u32(uint32_t* d_dst, const uint32_t* d_src, uint64_t n, void* stream) {
     if (n && (!d_dst || !d_src)) return fail(GPS_ERR_INVALID, "null pointer");
     if ((reinterpret_cast<uintptr_t>(d_dst) | reinterpret_cast<uintptr_t>(d_src)) & 3u)

@@ -63,6 +63,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t 
 hipError_t launch_copy_u32(const uint32_t* in, uint32_t* out, uint64_t n, hipStream_t stream);
 hipError_t launch_fill_splitmix_u32(uint32_t* p, uint64_t n, uint64_t seed, uint64_t first, uint32_t bits,
                                     hipStream_t stream);
+// Wave64 primitives over runs of 64 consecutive elements (device unit test).
+hipError_t launch_wave_unit_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t op, hipStream_t stream);
 hipError_t launch_validate_ones_u32(const uint32_t* scan, uint64_t n, int inclusive,
                                     unsigned long long* errors, hipStream_t stream);
 hipError_t launch_validate_diff_u32(const uint32_t* in, const uint32_t* scan, uint64_t n, int inclusive,

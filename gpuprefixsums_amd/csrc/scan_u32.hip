@@ -128,7 +128,7 @@ csdl_pp_u32(ScanParams p) {
     __shared__ uint32_t s_wave[C::kWaves];
     __shared__ uint32_t s_ticket[3];
     __shared__ uint32_t s_prefix;
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_index();
     const uint32_t lane = lane_id();
     const uint32_t total_tickets = p.num_tiles + gridDim.x;
 
@@ -630,6 +630,24 @@ __global__ void fill_splitmix_u32_kernel(uint32_t* p, uint64_t n, uint64_t seed,
         p[i] = (uint32_t)(splitmix64_at(seed, first + i) >> shift);
 }
 
+// Device unit kernel for the wave64 primitives (the reference's warp
+// primitives, GPUPrefixSumsCUDA/Utils.cuh:22-71): every run of 64
+// consecutive elements is one wave's lanes, a tail run zero-padded. The loop
+// bound is per wave, so all 64 lanes are active at the DPP steps.
+// op 0: inclusive scan, 1: run total in every lane, 2: exclusive scan.
+__global__ void wave_unit_kernel(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t op) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t lane = lane_id();
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1u)); g < n; g += stride) {
+        const uint64_t i = g + lane;
+        const uint32_t v = in[i < n ? i : g];
+        const uint32_t x = i < n ? v : 0u;
+        const uint32_t s = wave_incl_scan(x);
+        const uint32_t r = op == 0u ? s : op == 1u ? readlane(s, 63) : s - x;
+        if (i < n) out[i] = r;
+    }
+}
+
 // UtilityKernels.cuh:22-38: count scan[i] != i+1 (inclusive) / != i (exclusive).
 __global__ void validate_ones_kernel(const uint32_t* scan, uint64_t n, uint32_t incl,
                                      unsigned long long* errors) {
@@ -841,6 +859,13 @@ hipError_t launch_fill_splitmix_u32(uint32_t* p, uint64_t n, uint64_t seed, uint
     if (bits < 1 || bits > 32) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fill_splitmix_u32_kernel, grid_for(n, 256), dim3(256), 0, stream, p, n, seed, first,
                        64u - bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_unit_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t op, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    if (op > 2u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(wave_unit_kernel, grid_for(n, 256), dim3(256), 0, stream, in, out, n, op);
     return hipGetLastError();
 }
 

@@ -197,6 +197,14 @@ int gps_validate_ones_u32(const uint32_t* d_scan, uint64_t n, int inclusive, voi
 int gps_validate_scan_u32(const uint32_t* d_in, const uint32_t* d_scan, uint64_t n, int inclusive,
                           void* stream, uint64_t* errors);
 
+/* ---- device unit kernel for the wave64 primitives -------------------------
+ * The reference's warp primitives (Utils.cuh:22-71: InclusiveWarpScan,
+ * WarpReduceSum) are wave64 DPP routines here. This runs them on every run of
+ * 64 consecutive elements (a tail run zero-padded) so tests can check them in
+ * isolation: inclusive scan, run total in every lane, or exclusive scan. */
+enum { GPS_WAVE_INCLUSIVE_SCAN = 0, GPS_WAVE_SUM = 1, GPS_WAVE_EXCLUSIVE_SCAN = 2 };
+int gps_wave_primitive_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int op, void* stream);
+
 /* ---- vendor comparator: rocPRIM's device scan ----------------------------
  * Replaces the reference's CUB comparator (CubDispatcher.cuh:32-76:
  * cub::DeviceScan::InclusiveSum with a two-phase temporary-storage query).

@@ -253,3 +253,24 @@ def test_dispatcher_test_all(algo, inclusive):
          gps.ALGO_CSDLDF: gps.EmulatedDeadlockingDispatcher}[algo](1 << 22)
     passed, total = d.test_all_inclusive() if inclusive else d.test_all_exclusive()
     assert total > 4000 and passed == total
+
+
+@pytest.mark.parametrize("op", ["inclusive", "sum", "exclusive"])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4096 + 17, (1 << 22) + 5])
+def test_wave64_primitives_against_serial_per_wave_scan(n, op):
+    """Device unit test of the wave64 DPP primitives (Utils.cuh:22-71
+    counterparts): each run of 64 consecutive elements is one wave."""
+    x = oracle.gen_u32(77 + n, n, 32)
+    d_in, d_out = to_dev(x), sentinel_u32(n + 64)
+    code = {"inclusive": gps.WAVE_INCLUSIVE_SCAN, "sum": gps.WAVE_SUM, "exclusive": gps.WAVE_EXCLUSIVE_SCAN}[op]
+    gps.wave_primitive_u32(d_in, d_out, n, code)
+    sync()
+    got = from_dev_u32(d_out)
+    assert np.all(got[n:n + 64] == 0xDEADBEEF), "wrote past n"
+    runs = np.zeros(-(-n // 64) * 64, dtype=np.uint64)
+    runs[:n] = x
+    runs = runs.reshape(-1, 64)
+    incl = np.cumsum(runs, axis=1) & 0xFFFFFFFF  # serial per-wave scan in uint64, wrapped
+    want = {"inclusive": incl, "sum": np.repeat(incl[:, -1:], 64, axis=1),
+            "exclusive": (incl - runs) & 0xFFFFFFFF}[op].reshape(-1)[:n].astype(np.uint32)
+    assert np.array_equal(got[:n], want), first_mismatch(got[:n], want)
