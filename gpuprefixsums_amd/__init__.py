@@ -145,6 +145,14 @@ class ScanHandle:
         check(load().gps_segmented_scan_u32_per_segment(self.handle, _ptr(d_in), _ptr(d_out), ptr, off.size - 1,
                                                         int(inclusive), _stream(stream)))
 
+    def segmented_fp(self, d_in, d_out, d_offsets, num_segments: int, n: int, bf16: bool = False,
+                     compensated: bool = True, stream=None) -> None:
+        """Batched segmented FP inclusive scan in one launch (FP32 or BF16-bits
+        input, FP32 output; d_offsets: device uint64[num_segments + 1])."""
+        fn = load().gps_segmented_scan_bf16_f32 if bf16 else load().gps_segmented_scan_f32
+        check(fn(self.handle, _ptr(d_in), _ptr(d_out), _ptr(d_offsets), num_segments, n, int(compensated),
+                 _stream(stream)))
+
     def fp_per_segment(self, d_in, d_out, h_offsets, bf16: bool = False, compensated: bool = True,
                        stream=None) -> None:
         """One FP inclusive scan launch per segment (h_offsets: host uint64 sequence)."""

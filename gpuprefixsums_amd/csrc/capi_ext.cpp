@@ -64,6 +64,28 @@ int scan_fp(gps_handle* h, const void* in, float* out, uint64_t n, bool bf16, in
     return GPS_OK;
 }
 
+int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offsets, uint64_t num_segments,
+               uint64_t n, bool bf16, int compensated, void* stream) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "floating-point scans need a CSDL handle");
+    if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
+    if (!offsets) return fail(GPS_ERR_INVALID, "null offsets");
+    if (n > h->max_n)
+        return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
+                    (unsigned long long)h->max_n);
+    const uintptr_t ia = reinterpret_cast<uintptr_t>(in), oa = reinterpret_cast<uintptr_t>(out);
+    if ((ia & (bf16 ? 1u : 3u)) || (oa & 3u)) return fail(GPS_ERR_INVALID, "misaligned buffer");
+    if (reinterpret_cast<uintptr_t>(offsets) & 7u) return fail(GPS_ERR_INVALID, "offsets must be 8-byte aligned");
+    if (n == 0) return GPS_OK;
+    const gps::TileShape s = gps::fp_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
+    const uint32_t vec_ok = ((ia & (bf16 ? 7u : 15u)) == 0) && ((oa & 15u) == 0);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(gps::before_epoch_launch(h, st));
+    HIP_TRY(gps::launch_segscan_fp(s, bf16, compensated != 0, in, out, offsets, num_segments, n, vec_ok, h->status,
+                                   h->ticket, h->err, st));
+    return GPS_OK;
+}
+
 }  // namespace
 
 struct gps_graph {
@@ -145,6 +167,16 @@ int gps_inclusive_scan_f32(gps_handle* h, const float* d_in, float* d_out, uint6
 int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, uint64_t n, int compensated,
                                 void* stream) {
     return scan_fp(h, d_in, d_out, n, true, compensated, stream);
+}
+
+int gps_segmented_scan_f32(gps_handle* h, const float* d_in, float* d_out, const uint64_t* d_offsets,
+                           uint64_t num_segments, uint64_t n, int compensated, void* stream) {
+    return segscan_fp(h, d_in, d_out, d_offsets, num_segments, n, false, compensated, stream);
+}
+
+int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, const uint64_t* d_offsets,
+                                uint64_t num_segments, uint64_t n, int compensated, void* stream) {
+    return segscan_fp(h, d_in, d_out, d_offsets, num_segments, n, true, compensated, stream);
 }
 
 int gps_inclusive_scan_fp_per_segment(gps_handle* h, const void* d_in, float* d_out, const uint64_t* h_offsets,

@@ -53,6 +53,10 @@ hipError_t launch_segscan_u32(TileShape s, bool inclusive, const uint32_t* in, u
 bool fp_shape_supported(TileShape s);
 hipError_t launch_scan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out, uint64_t n,
                           uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
+// Batched segmented FP inclusive scan (one launch; same shapes as launch_scan_fp).
+hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out,
+                             const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,
+                             uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
 
 // Clears `count` status words and arms the ticket word at epoch 1.
 hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream);

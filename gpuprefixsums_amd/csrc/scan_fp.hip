@@ -16,6 +16,7 @@
 // This file is compiled with -ffp-contract=off: a fused multiply-add would
 // break the TwoSum error terms.
 #include "gps_internal.hpp"
+#include "segments.hpp"
 #include "tile_u32.hpp"
 
 namespace gps {
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(BLOCK) void scan_fp(FpParams p) {
     __shared__ float s_ws[C::kWaves], s_wc[C::kWaves];
     __shared__ uint32_t s_ticket[2];
     __shared__ float s_ps, s_pc;
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_index();
     const uint32_t lane = lane_id();
     if (threadIdx.x == 0) {
         const Ticket t = take_ticket(p.ticket, p.num_tiles);
@@ -267,6 +268,184 @@ __global__ __launch_bounds__(BLOCK) void scan_fp(FpParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched segmented FP inclusive scan: one launch over concatenated segments
+// (segment s covers [offsets[s], offsets[s+1])), heads found as in the uint32
+// segmented scan (segments.hpp). A floating-point carry cannot be cut at a
+// head by subtracting prefixes (cancellation), so every level applies the
+// segmented operator to (flag, pair):
+//     (f1, v1) + (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2)
+// in the lane, in the DPP wave scan, across vectors, across waves and along
+// the lookback chain. A tile that contains a head publishes INCLUSIVE at once
+// (its carry-out is the sum since its last head); only elements before the
+// tile's first head take the looked-back prefix.
+// ---------------------------------------------------------------------------
+#define GPS_DPP_I(x, CTRL, RM, BC) ((uint32_t)__builtin_amdgcn_update_dpp(0, (int)(x), CTRL, RM, 0xf, BC))
+
+// Inclusive segmented wave scan of (f, x). Lanes that a DPP step does not
+// feed (bound_ctrl, or rows outside the broadcast mask) receive (0, 0), the
+// identity, so the same six-step schedule as the plain scan applies.
+template <bool COMP>
+__device__ __forceinline__ void wave_seg_scan_ff(uint32_t& f, ff& x) {
+#define GPS_SEG_STEP(CTRL, RM, BC)                                                                  \
+    {                                                                                               \
+        const ff y = {GPS_DPP_F(x.s, CTRL, RM, BC), COMP ? GPS_DPP_F(x.c, CTRL, RM, BC) : 0.f};     \
+        const uint32_t yf = GPS_DPP_I(f, CTRL, RM, BC);                                             \
+        const ff sum = ff_add<COMP>(y, x);                                                          \
+        x = f ? x : sum;                                                                            \
+        f |= yf;                                                                                    \
+    }
+    GPS_SEG_STEP(0x111, 0xf, true)
+    GPS_SEG_STEP(0x112, 0xf, true)
+    GPS_SEG_STEP(0x114, 0xf, true)
+    GPS_SEG_STEP(0x118, 0xf, true)
+    GPS_SEG_STEP(0x142, 0xa, false)
+    GPS_SEG_STEP(0x143, 0xc, false)
+#undef GPS_SEG_STEP
+}
+
+// acc <- acc (+) (f, v) under the segmented operator (acc on the left).
+template <bool COMP>
+__device__ __forceinline__ void seg_combine_ff(uint32_t& acc_f, ff& acc, uint32_t f, ff v) {
+    acc = f ? v : ff_add<COMP>(acc, v);
+    acc_f |= f;
+}
+
+struct SegFpParams {
+    const void* in;
+    float* out;
+    const uint64_t* offsets;
+    uint64_t num_segments;
+    uint64_t n;
+    uint32_t num_tiles;
+    uint32_t vec_ok;
+    uint64_t* status;  // 2 words per tile
+    uint64_t* ticket;
+    uint32_t* err;
+};
+
+template <int BLOCK, int VEC, bool BF16, bool COMP>
+__global__ __launch_bounds__(BLOCK) void segscan_fp(SegFpParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    constexpr uint32_t kWords = C::kTile / 32;
+    __shared__ uint32_t s_heads[kWords];
+    __shared__ float s_ws[C::kWaves], s_wc[C::kWaves];
+    __shared__ uint32_t s_wf[C::kWaves];
+    __shared__ uint32_t s_ticket[2];
+    __shared__ float s_ps, s_pc;
+    __shared__ uint64_t s_first_seg;
+    const uint32_t wave = wave_index();
+    const uint32_t lane = lane_id();
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, p.num_tiles);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    for (uint32_t i = threadIdx.x; i < kWords; i += BLOCK) s_heads[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_ticket[0], epoch = s_ticket[1];
+    const uint64_t a = (uint64_t)tile * C::kTile;
+    const uint64_t b = a + C::kTile < p.n ? a + C::kTile : p.n;
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    const bool vec_path = p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n);
+    f32x4 x[VEC];
+    load_fp<BF16, VEC>(x, p.in, base, p.n, vec_path);  // the head search overlaps these loads
+
+    if (wave == 0) {
+        const uint64_t s0 = find_first_segment(p.offsets, p.num_segments, a);
+        if (lane == 0) s_first_seg = s0;
+    }
+    __syncthreads();
+    mark_heads<BLOCK>(s_heads, p.offsets, p.num_segments, s_first_seg, a, b, tile == 0);
+    __syncthreads();
+
+    // Per vector: lane-local segmented sums, the segmented wave scan, and the
+    // lane's carry-in. `open` bit k: no head precedes vector k's lane in the
+    // wave's run, so the carry from earlier waves / tiles still applies.
+    const uint32_t run_base = wave * C::kWaveElems;
+    uint32_t nibs[VEC];
+    ff lane_carry[VEC];
+    uint32_t open = 0;
+    ff run = {0.f, 0.f};
+    uint32_t run_f = 0;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        const uint32_t nib = head_nibble(s_heads, run_base + (uint32_t)k * 256u + lane * 4u);
+        nibs[k] = nib;
+        ff t = {x[k].x, 0.f};
+        t = (nib & 2u) ? ff{x[k].y, 0.f} : ff_add1<COMP>(t, x[k].y);
+        t = (nib & 4u) ? ff{x[k].z, 0.f} : ff_add1<COMP>(t, x[k].z);
+        t = (nib & 8u) ? ff{x[k].w, 0.f} : ff_add1<COMP>(t, x[k].w);
+        uint32_t sf = nib != 0u;
+        ff sv = t;
+        wave_seg_scan_ff<COMP>(sf, sv);
+        const ff ev = {GPS_DPP_F(sv.s, 0x138, 0xf, true), COMP ? GPS_DPP_F(sv.c, 0x138, 0xf, true) : 0.f};
+        const uint32_t ef = GPS_DPP_I(sf, 0x138, 0xf, true);  // wave_shr:1, lane 0 gets (0, 0)
+        lane_carry[k] = ef ? ev : ff_add<COMP>(run, ev);
+        open |= (ef | run_f) ? 0u : (1u << k);
+        const uint32_t tf = readlane(sf, 63);
+        const ff tv = {readlane_f(sv.s, 63), COMP ? readlane_f(sv.c, 63) : 0.f};
+        seg_combine_ff<COMP>(run_f, run, tf, tv);
+    }
+    if (lane == 0) {
+        s_ws[wave] = run.s;
+        s_wc[wave] = run.c;
+        s_wf[wave] = run_f;
+    }
+    __syncthreads();
+    uint32_t pre_f = 0, tile_f = 0;
+    ff pre = {0.f, 0.f}, tile_tot = {0.f, 0.f};
+#pragma unroll
+    for (uint32_t q = 0; q < C::kWaves; ++q) {
+        const ff v = {s_ws[q], s_wc[q]};
+        const uint32_t fq = s_wf[q];
+        if (q < wave) seg_combine_ff<COMP>(pre_f, pre, fq, v);
+        seg_combine_ff<COMP>(tile_f, tile_tot, fq, v);
+    }
+    const bool starts_with_head = (s_heads[0] & 1u) != 0u;
+    if (wave == 0) {
+        if (lane == 0)
+            publish_ff<COMP>(p.status, tile, tile_tot, (tile_f || tile == 0) ? kFlagIncl : kFlagAgg, epoch);
+        ff prefix = {0.f, 0.f};
+        if (tile != 0 && !starts_with_head) {
+            prefix = lookback_ff<COMP>(p.status, tile, epoch, p.err);
+            if (!tile_f && lane == 0)
+                publish_ff<COMP>(p.status, tile, ff_add<COMP>(prefix, tile_tot), kFlagIncl, epoch);
+        }
+        if (lane == 0) {
+            s_ps = prefix.s;
+            s_pc = prefix.c;
+        }
+    }
+    __syncthreads();
+    const ff run_in = pre_f ? pre : ff_add<COMP>({s_ps, s_pc}, pre);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        ff acc = (open >> k) & 1u ? ff_add<COMP>(run_in, lane_carry[k]) : lane_carry[k];
+        f32x4 y;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if ((nibs[k] >> c) & 1u) acc = {0.f, 0.f};
+            acc = ff_add1<COMP>(acc, x[k][c]);
+            y[c] = acc.s + acc.c;
+        }
+        x[k] = y;
+    }
+    if (vec_path) {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k)
+            __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
+    } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                if (i < p.n) p.out[i] = x[k][c];
+            }
+    }
+}
+
 #define GPS_FP_SHAPES(X) \
     X(256, 4)            \
     X(1024, 4)           \
@@ -302,6 +481,37 @@ hipError_t launch_scan_fp(TileShape s, bool bf16, bool compensated, const void* 
         else                                                                                                    \
             hipLaunchKernelGGL((scan_fp<B, V, false, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);     \
         return hipGetLastError();                                                                               \
+    }
+    GPS_FP_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out,
+                             const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,
+                             uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream) {
+    SegFpParams p{};
+    p.in = in;
+    p.out = out;
+    p.offsets = offsets;
+    p.num_segments = num_segments;
+    p.n = n;
+    p.num_tiles = (uint32_t)((n + s.tile() - 1) / s.tile());
+    p.vec_ok = vec_ok;
+    p.status = status;
+    p.ticket = ticket;
+    p.err = err;
+#define X(B, V)                                                                                                  \
+    if (s.block == B && s.vec == V) {                                                                            \
+        if (bf16 && compensated)                                                                                 \
+            hipLaunchKernelGGL((segscan_fp<B, V, true, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);     \
+        else if (bf16)                                                                                           \
+            hipLaunchKernelGGL((segscan_fp<B, V, true, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);    \
+        else if (compensated)                                                                                    \
+            hipLaunchKernelGGL((segscan_fp<B, V, false, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);    \
+        else                                                                                                     \
+            hipLaunchKernelGGL((segscan_fp<B, V, false, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);   \
+        return hipGetLastError();                                                                                \
     }
     GPS_FP_SHAPES(X)
 #undef X

@@ -30,7 +30,7 @@ struct TileFront {
     bool vec_path;
 
     __device__ __forceinline__ void run(const ScanParams& p, uint32_t* s_wave, uint32_t* s_ticket) {
-        const uint32_t wave = threadIdx.x / kWave;
+        const uint32_t wave = wave_index();
         if (threadIdx.x == 0) {
             const Ticket t = take_ticket(p.ticket, p.num_tiles);
             s_ticket[0] = t.tile;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES
     TileFront<BLOCK, VEC, INCL> f;
     f.run(p, s_wave, s_ticket);
 
-    if (threadIdx.x < kWave) {
+    if (wave_index() == 0) {
         const uint32_t lane = lane_id();
         if (f.tile == 0) {
             if (lane == 0) {
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
     TileFront<BLOCK, VEC, INCL> f;
     f.run(p, s_wave, s_ticket);
     const uint32_t tile = f.tile, epoch = f.epoch;
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_index();
     const uint32_t lane = lane_id();
 
     if (threadIdx.x == 0) {

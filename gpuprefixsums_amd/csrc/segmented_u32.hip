@@ -19,32 +19,10 @@
 //   - only elements before the tile's first head take the looked-back carry.
 // All arithmetic is mod 2^32, bit-exact with a serial per-segment loop.
 #include "gps_internal.hpp"
+#include "segments.hpp"
 #include "tile_u32.hpp"
 
 namespace gps {
-
-// First index s in [0, S) with offsets[s] >= a (S if none). One full wave;
-// returns the same value in every lane.
-__device__ __forceinline__ uint64_t find_first_segment(const uint64_t* offsets, uint64_t S, uint64_t a) {
-    const uint32_t lane = lane_id();
-    uint64_t lo = 0, hi = S;  // answer in [lo, hi]
-    while (hi - lo > 64) {
-        const uint64_t step = (hi - lo + 63) / 64;
-        uint64_t pos = lo + (uint64_t)(lane + 1) * step - 1;
-        if (pos > hi - 1) pos = hi - 1;
-        const uint64_t hit = ballot(offsets[pos] >= a);
-        if (!hit) return hi;  // every sample (including hi - 1) is below a
-        const uint32_t f = (uint32_t)__builtin_ctzll(hit);
-        uint64_t prev = lo + (uint64_t)f * step - 1;  // pos of lane f - 1
-        if (prev > hi - 1) prev = hi - 1;
-        const uint64_t new_lo = f == 0 ? lo : prev + 1;
-        hi = lo + (uint64_t)(f + 1) * step - 1 < hi - 1 ? lo + (uint64_t)(f + 1) * step - 1 : hi - 1;
-        lo = new_lo;
-    }
-    const uint64_t pos = lo + lane;
-    const uint64_t hit = ballot(pos < hi && offsets[pos] >= a);
-    return hit ? lo + (uint64_t)__builtin_ctzll(hit) : hi;
-}
 
 struct SegParams {
     const uint32_t* in;
@@ -76,7 +54,7 @@ __global__ __launch_bounds__(BLOCK) void segscan_u32(SegParams p) {
     __shared__ uint32_t s_prefix;
     __shared__ uint64_t s_first_seg;
 
-    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t wave = wave_index();
     const uint32_t lane = lane_id();
     if (threadIdx.x == 0) {
         const Ticket t = take_ticket(p.ticket, p.num_tiles);
@@ -103,29 +81,7 @@ __global__ __launch_bounds__(BLOCK) void segscan_u32(SegParams p) {
         if (lane == 0) s_first_seg = s0;
     }
     __syncthreads();
-    {
-        const uint64_t s0 = s_first_seg;
-        if (tile == 0 && threadIdx.x == 0) atomicOr(&s_heads[0], 1u);  // position 0 starts a segment
-        for (uint64_t s = s0 + threadIdx.x; s < p.num_segments; s += 4u * BLOCK) {
-            uint64_t o[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint64_t su = s + (uint64_t)u * BLOCK;
-                o[u] = su < p.num_segments ? p.offsets[su] : b;
-            }
-            bool past = false;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (o[u] < b) {
-                    const uint32_t rel = (uint32_t)(o[u] - a);
-                    atomicOr(&s_heads[rel >> 5], 1u << (rel & 31u));
-                } else {
-                    past = true;
-                }
-            }
-            if (past) break;
-        }
-    }
+    mark_heads<BLOCK>(s_heads, p.offsets, p.num_segments, s_first_seg, a, b, tile == 0);
     __syncthreads();
 
     // Segmented scan of the wave's run (carry-in 0). For each element keep
@@ -139,7 +95,7 @@ __global__ __launch_bounds__(BLOCK) void segscan_u32(SegParams p) {
     for (int k = 0; k < VEC; ++k) {
         if (!INCL) x[k] = d[k];
         const uint32_t e = run_base + (uint32_t)k * 256u + lane * 4u;  // tile-relative index of component 0
-        const uint32_t nib = (s_heads[e >> 5] >> (e & 31u)) & 0xFu;
+        const uint32_t nib = head_nibble(s_heads, e);
         // lane-local segmented inclusive scan
         u32x4 v = d[k];
         v.y = (nib & 2u) ? v.y : v.x + v.y;

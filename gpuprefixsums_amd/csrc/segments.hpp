@@ -1,0 +1,73 @@
+// Segment-head discovery shared by the batched segmented scans (uint32 and
+// floating point). Segment s covers [offsets[s], offsets[s+1]); a tile
+// covering [a, b) marks every offset that falls inside it in an LDS bitmap
+// with one bit per tile element.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace gps {
+
+// First index s in [0, S) with offsets[s] >= a (S if none), by a 64-ary
+// search: each round one wave samples 64 evenly spaced offsets and keeps the
+// bracket around the first sample >= a. One full wave; returns the same
+// value in every lane. Loads are branch-free (lanes past the bracket reload
+// a valid entry), so no divergent region precedes a ballot.
+__device__ __forceinline__ uint64_t find_first_segment(const uint64_t* offsets, uint64_t S, uint64_t a) {
+    const uint32_t lane = lane_id();
+    uint64_t lo = 0, hi = S;  // answer in [lo, hi]
+    while (hi - lo > 64) {
+        const uint64_t step = (hi - lo + 63) / 64;
+        uint64_t pos = lo + (uint64_t)(lane + 1) * step - 1;
+        if (pos > hi - 1) pos = hi - 1;
+        const uint64_t hit = ballot(offsets[pos] >= a);
+        if (!hit) return hi;  // every sample (including hi - 1) is below a
+        const uint32_t f = (uint32_t)__builtin_ctzll(hit);
+        uint64_t prev = lo + (uint64_t)f * step - 1;  // pos of lane f - 1
+        if (prev > hi - 1) prev = hi - 1;
+        const uint64_t new_lo = f == 0 ? lo : prev + 1;
+        hi = lo + (uint64_t)(f + 1) * step - 1 < hi - 1 ? lo + (uint64_t)(f + 1) * step - 1 : hi - 1;
+        lo = new_lo;
+    }
+    // lanes past hi reload offsets[lo], which exists since lo <= S
+    const uint64_t pos = lo + lane;
+    const bool ge = offsets[pos < hi ? pos : lo] >= a;
+    const uint64_t hit = ballot(pos < hi && ge);
+    return hit ? lo + (uint64_t)__builtin_ctzll(hit) : hi;
+}
+
+// Sets bit (o - a) of s_heads for every offset o in [a, b), scanning the
+// offsets from index s0 (the first one >= a) with the whole workgroup; bit 0
+// of tile 0 is always set (position 0 starts a segment). s_heads must be
+// zeroed and s0 visible to all threads (barriers are the caller's).
+template <int BLOCK>
+__device__ __forceinline__ void mark_heads(uint32_t* s_heads, const uint64_t* offsets, uint64_t S, uint64_t s0,
+                                           uint64_t a, uint64_t b, bool first_tile) {
+    if (first_tile && threadIdx.x == 0) atomicOr(&s_heads[0], 1u);
+    for (uint64_t s = s0 + threadIdx.x; s < S; s += 4u * BLOCK) {
+        uint64_t o[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t su = s + (uint64_t)u * BLOCK;
+            o[u] = su < S ? offsets[su] : b;
+        }
+        bool past = false;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (o[u] < b) {
+                const uint32_t rel = (uint32_t)(o[u] - a);
+                atomicOr(&s_heads[rel >> 5], 1u << (rel & 31u));
+            } else {
+                past = true;
+            }
+        }
+        if (past) break;
+    }
+}
+
+// The 4 head bits of the elements e .. e+3 (e a multiple of 4).
+__device__ __forceinline__ uint32_t head_nibble(const uint32_t* s_heads, uint32_t e) {
+    return (s_heads[e >> 5] >> (e & 31u)) & 0xFu;
+}
+
+}  // namespace gps

@@ -50,7 +50,7 @@ def best_static(ws: List[Workload], times: Dict[int, Dict[str, float]]) -> Dict[
     out = {}
     for dt in sorted({w.dtype for w in ws}):
         sub = [w for w in ws if w.dtype == dt]
-        names = [c.name for c in candidates(dt)]
+        names = [c.name for c in candidates(dt) if all(c.name in times[w.wid] for w in sub)]
         mean = {c: np.mean([times[w.wid][c] / min(times[w.wid].values()) - 1.0 for w in sub]) for c in names}
         out[dt] = min(mean, key=mean.get)
     return out

@@ -48,7 +48,7 @@ class Runner:
         h = self.handles.get(c)
         if h is None:
             algo = {"batched": gps.ALGO_CSDL, "loop_csdl": gps.ALGO_CSDL, "loop_rts": gps.ALGO_RTS,
-                    "loop_csdldf": gps.ALGO_CSDLDF, "loop_fp": gps.ALGO_CSDL}[c.kind]
+                    "loop_csdldf": gps.ALGO_CSDLDF, "loop_fp": gps.ALGO_CSDL, "batched_fp": gps.ALGO_CSDL}[c.kind]
             h = gps.ScanHandle(algo, self.max_n, block_threads=c.block, vec_per_thread=c.vec,
                                spin_limit=c.spin if c.kind == "loop_csdldf" else 0)
             self.handles[c] = h
@@ -67,7 +67,10 @@ class Runner:
                 h.segmented_per_segment(self.u32, self.out, w.offsets, False)
         else:
             src = self.f32 if w.dtype == "f32" else self.bf16
-            h.fp_per_segment(src, self.out, w.offsets, bf16=w.dtype == "bf16", compensated=True)
+            if c.kind == "batched_fp":
+                h.segmented_fp(src, self.out, self.off, w.segments, w.n, bf16=w.dtype == "bf16", compensated=True)
+            else:
+                h.fp_per_segment(src, self.out, w.offsets, bf16=w.dtype == "bf16", compensated=True)
 
     def time(self, w: Workload, c: Config, best_so_far: float = float("inf")) -> float:
         torch = self.torch

@@ -9,6 +9,7 @@ configuration fixes how it runs:
   loop_rts     one reduce-then-scan launch per segment (u32)   tile shape
   loop_csdldf  one decoupled-fallback launch per segment (u32) spin limit
   loop_fp      one compensated FP scan launch per segment      tile shape
+  batched_fp   the one-launch segmented FP kernel              tile shape
 
 Reference knobs these generalise: CSDL_WARPS / UINT4_PER_THREAD
 (GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:13-14) and
@@ -23,7 +24,7 @@ from typing import List
 import numpy as np
 
 DTYPES = ("u32", "f32", "bf16")
-KINDS = ("batched", "loop_csdl", "loop_rts", "loop_csdldf", "loop_fp")
+KINDS = ("batched", "loop_csdl", "loop_rts", "loop_csdldf", "loop_fp", "batched_fp")
 
 
 @dataclass(frozen=True)
@@ -52,7 +53,8 @@ U32_CONFIGS = (
     + [Config("loop_rts", b, v) for b, v in ((512, 2), (256, 4), (1024, 2))]
     + [Config("loop_csdldf", 1024, 4, s) for s in (1, 4, 32)]
 )
-FP_CONFIGS = [Config("loop_fp", b, v) for b, v in ((1024, 8), (1024, 4), (256, 4))]
+FP_SHAPES = ((1024, 8), (1024, 4), (256, 4))
+FP_CONFIGS = [Config("loop_fp", b, v) for b, v in FP_SHAPES] + [Config("batched_fp", b, v) for b, v in FP_SHAPES]
 
 
 def candidates(dtype: str) -> List[Config]:
@@ -61,10 +63,10 @@ def candidates(dtype: str) -> List[Config]:
 
 def default_config(dtype: str, segments: int) -> Config:
     """What the library does without the tuner: the default CSDL shape for a
-    single scan, the default batched kernel for several segments, the default
-    FP shape for floating point."""
+    single scan, the default batched kernel for several segments, and for
+    floating point the default FP shape (one launch, plain or segmented)."""
     if dtype != "u32":
-        return Config("loop_fp", 1024, 8)
+        return Config("loop_fp", 1024, 8) if segments == 1 else Config("batched_fp", 1024, 8)
     return Config("loop_csdl", 1024, 8) if segments == 1 else Config("batched", 1024, 4)
 
 

@@ -154,8 +154,19 @@ int gps_mlp_loss_grad(gps_mlp* m, const float* h_x, const float* h_t, uint32_t b
 int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch, uint32_t epochs, float lr,
                   float weight_decay, float* h_loss_curve);
 
+/* Batched segmented FP inclusive scan in one launch over the concatenated
+ * segments: segment s covers [d_offsets[s], d_offsets[s+1]) (device array of
+ * num_segments + 1 entries, d_offsets[0] = 0, d_offsets[S] = n; empty
+ * segments allowed). Same input/output types and compensation as the plain
+ * FP scans; the carry restarts at every segment head at every level (lane,
+ * wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. */
+int gps_segmented_scan_f32(gps_handle* h, const float* d_in, float* d_out, const uint64_t* d_offsets,
+                           uint64_t num_segments, uint64_t n, int compensated, void* stream);
+int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, const uint64_t* d_offsets,
+                                uint64_t num_segments, uint64_t n, int compensated, void* stream);
+
 /* One FP scan launch per non-empty segment (host offsets; d_in is float or
- * bf16 per `bf16`): the batched FP form, used by the tuner's workloads. */
+ * bf16 per `bf16`): the per-segment baseline of the batched FP form. */
 int gps_inclusive_scan_fp_per_segment(gps_handle* h, const void* d_in, float* d_out, const uint64_t* h_offsets,
                                       uint64_t num_segments, int bf16, int compensated, void* stream);
 

@@ -13,6 +13,7 @@ What each function restates (reference files relative to the reference root):
                             GPUPrefixSumsCUDA/UtilityKernels.cuh:22-38
   segmented_scan_u32        per-segment serial loop (no reference counterpart)
   scan_f32_f64 / bf16       FP64 Neumaier-compensated running sum (no counterpart)
+  segmented_scan_fp_f64     the FP64 scans restarted at every segment offset
   generators                counter-based splitmix64 streams (no counterpart)
 
 How each is pinned (tests/test_oracle.py): closed forms, brute force with
@@ -164,6 +165,23 @@ def abs_scan_bf16_f64(bits) -> np.ndarray:
     bits = _c(bits, np.uint16)
     out = np.empty(bits.size, np.float64)
     lib().orc_abs_scan_bf16_f64(bits, out, bits.size)
+    return out
+
+
+def segmented_scan_fp_f64(x, offsets, bf16: bool = False, absolute: bool = False) -> np.ndarray:
+    """Per-segment FP64 inclusive scan (or scan of |x|) of FP32 values or BF16
+    bits: the scans above restarted at every offset (no reference counterpart)."""
+    offsets = _c(offsets, np.uint64)
+    assert offsets.size >= 1 and int(offsets[0]) == 0 and int(offsets[-1]) == np.asarray(x).size
+    f = (abs_scan_bf16_f64 if absolute else scan_bf16_f64) if bf16 else \
+        (abs_scan_f32_f64 if absolute else scan_f32_f64)
+    out = np.empty(np.asarray(x).size, np.float64)
+    for s in range(offsets.size - 1):
+        a, b = int(offsets[s]), int(offsets[s + 1])
+        if b < a:
+            raise ValueError("bad segment offsets")
+        if b > a:
+            out[a:b] = f(x[a:b])
     return out
 
 

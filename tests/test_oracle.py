@@ -210,6 +210,23 @@ def test_fp64_scan_against_fsum():
     assert np.allclose(oracle.abs_scan_f32_f64(x), np.cumsum(np.abs(x.astype(np.float64))))
 
 
+def test_segmented_fp64_scan_against_fsum_per_segment():
+    x = oracle.gen_f32_normal(8, 500)
+    off = np.array([0, 0, 1, 1, 40, 41, 300, 500, 500], dtype=np.uint64)  # empty and length-1 segments
+    y = oracle.segmented_scan_fp_f64(x, off)
+    a_ = oracle.segmented_scan_fp_f64(x, off, absolute=True)
+    for s in range(off.size - 1):
+        a, b = int(off[s]), int(off[s + 1])
+        for i in range(a, b):
+            assert y[i] == math.fsum(float(v) for v in x[a:i + 1])
+            assert a_[i] == pytest.approx(math.fsum(abs(float(v)) for v in x[a:i + 1]), rel=1e-15)
+    bits = oracle.gen_bf16_logmag(5, 300)
+    yb = oracle.segmented_scan_fp_f64(bits, np.array([0, 150, 300], dtype=np.uint64), bf16=True)
+    assert yb[150] == float(oracle.bf16_bits_to_f32(bits[150:151])[0])  # restarts at the head
+    with pytest.raises(ValueError):
+        oracle.segmented_scan_fp_f64(x, np.array([0, 300, 200, 500], dtype=np.uint64))
+
+
 def test_bf16_generator_magnitudes():
     bits = oracle.gen_bf16_logmag(5, 20000)
     v = np.abs(oracle.bf16_bits_to_f32(bits).astype(np.float64))

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from gpuprefixsums_amd.tuner.evaluate import best_static, regrets
-from gpuprefixsums_amd.tuner.space import (FEATURES, Config, Workload, c5_workloads, candidates, default_config,
+from gpuprefixsums_amd.tuner.space import (FEATURES, KINDS, Config, Workload, c5_workloads, candidates, default_config,
                                            features, lognormal_partition, split)
 
 
@@ -43,13 +43,14 @@ def test_config_names_round_trip_and_candidates():
             assert Config.parse(c.name) == c
         assert default_config(dt, 1).name in {c.name for c in cs}
         assert default_config(dt, 100).name in {c.name for c in cs}
-    assert all(c.kind == "loop_fp" for c in candidates("f32"))
+    assert {c.kind for c in candidates("f32")} == {"loop_fp", "batched_fp"}
+    assert default_config("bf16", 100).kind == "batched_fp" and default_config("u32", 100).kind == "batched"
 
 
 def test_features_dimension_and_values():
     w = Workload(0, 1 << 20, "bf16", 4, np.array([0, 1, 2, 3, 1 << 20], dtype=np.uint64))
     f = features(w, Config("loop_csdldf", 1024, 4, 32))
-    assert len(f) == len(FEATURES) == 15
+    assert len(f) == len(FEATURES) == 7 + len(KINDS) + 3
     d = dict(zip(FEATURES, f))
     assert d["log2_n"] == 20 and d["log2_segments"] == 2
     assert d["is_bf16"] == 1 and d["is_u32"] == 0
@@ -64,20 +65,22 @@ class _StubModel:
         self.names = []
 
     def predict(self, X):
-        # features() puts the kind one-hot and the shape last; recover the name
+        # recover each row's configuration from its kind one-hot and shape features
+        col = {name: i for i, name in enumerate(FEATURES)}
         out = []
         for row in np.asarray(X):
-            kind = [k for k, v in zip(("batched", "loop_csdl", "loop_rts", "loop_csdldf", "loop_fp"), row[7:12])
-                    if v == 1][0]
-            c = Config(kind, int(round(2 ** row[12])), int(round(2 ** row[13])),
-                       int(round(2 ** row[14] - 1)) if kind == "loop_csdldf" else 0)
+            kind = [k for k in KINDS if row[col[f"kind_{k}"]] == 1][0]
+            c = Config(kind, int(round(2 ** row[col["log2_block"]])), int(round(2 ** row[col["log2_vec"]])),
+                       int(round(2 ** row[col["log2_spin"]] - 1)) if kind == "loop_csdldf" else 0)
             out.append(self.table.get(c.name, 100.0))
         return np.array(out)
 
 
 def test_regret_arithmetic_against_hand_computed_values():
-    ws = [Workload(i, 4096, "f32", 2, np.array([0, 2048, 4096], dtype=np.uint64)) for i in range(4)]
-    names = [c.name for c in candidates("f32")]  # 1024x8, 1024x4, 256x4
+    # single-segment f32 workloads timed on the three one-launch loop_fp shapes
+    ws = [Workload(i, 4096, "f32", 1, np.array([0, 4096], dtype=np.uint64)) for i in range(4)]
+    names = [c.name for c in candidates("f32") if c.kind == "loop_fp"]  # 1024x8, 1024x4, 256x4
+    assert default_config("f32", 1).name == names[0]
     times = {
         0: {names[0]: 2.0, names[1]: 1.0, names[2]: 4.0},
         1: {names[0]: 3.0, names[1]: 1.5, names[2]: 1.0},
