@@ -14,7 +14,8 @@ Extra legs (reported under "extras"): csdl_exclusive, rts, csdldf (stall
 periods 0/7/8), copy (the read-n + write-n ceiling), vendor (rocPRIM's device
 scan, the reference's CUB comparator on ROCm), n2p20 (latency at 2^20),
 segmented (4096 log-normal segments: one graph-launched pass vs one launch
-per segment), fp (n = 2^27 FP32/BF16 with and without compensated carries,
+per segment), segmented_fp (the same segments over FP32: one segmented FP
+launch vs one launch per segment), fp (n = 2^27 FP32/BF16 with and without compensated carries,
 errors vs FP64), tuner (held-out regret of the learned cost model).
 """
 from __future__ import annotations
@@ -33,7 +34,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
-ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "fp", "tuner"]
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
+            "tuner"]
 
 
 def log(msg: str) -> None:
@@ -174,6 +176,53 @@ class Legs:
             "segments": S, "n_total": n, "graph_ms": round(graph_ms, 4), "per_segment_launches_ms": round(loop_ms, 3),
             "speedup": round(loop_ms / graph_ms, 1), "graph_gbps": round(nbytes / graph_ms / 1e6, 1),
             "per_segment_gbps": round(nbytes / loop_ms / 1e6, 1), "bytes": nbytes}
+
+    def segmented_fp(self, out):
+        """The same 4096 log-normal segments (seed 3) over FP32 ~ N(0,1)
+        values (seed 4), compensated inclusive: one segmented FP launch vs one
+        FP scan launch per segment; errors of both against the FP64 oracle."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        S = 4096
+        off = oracle.gen_lognormal_offsets(3, S, median=4096.0, sigma=1.5, lo=1, hi=1 << 22)
+        n = int(off[-1])
+        xh = oracle.gen_f32_normal(4, n)
+        x = torch.from_numpy(xh).to("cuda")
+        d_off = torch.from_numpy(off.view(np.int64)).to("cuda")
+        a = torch.empty_like(x)
+        b = torch.empty_like(x)
+        st = torch.cuda.current_stream()
+        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+        with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+            for _ in range(3):
+                h.segmented_fp(x, a, d_off, S, n)
+            e0, e1 = ev(), ev()
+            e0.record(st)
+            for _ in range(20):
+                h.segmented_fp(x, a, d_off, S, n)
+            e1.record(st)
+            torch.cuda.synchronize()
+            batched_ms = e0.elapsed_time(e1) / 20
+            h.fp_per_segment(x, b, off)
+            torch.cuda.synchronize()
+            e0, e1 = ev(), ev()
+            e0.record(st)
+            for _ in range(3):
+                h.fp_per_segment(x, b, off)
+            e1.record(st)
+            torch.cuda.synchronize()
+            loop_ms = e0.elapsed_time(e1) / 3
+            if h.device_error():
+                raise RuntimeError("device error in the segmented FP leg")
+        y64 = oracle.segmented_scan_fp_f64(xh, off)
+        rel = lambda y: np.abs(y.cpu().numpy().astype(np.float64) - y64) / np.maximum(np.abs(y64), 1.0)  # noqa: E731
+        ra, rb = rel(a), rel(b)
+        nbytes = 8 * n + 8 * (S + 1)
+        out["segmented_fp"] = {
+            "segments": S, "n_total": n, "batched_ms": round(batched_ms, 4),
+            "per_segment_launches_ms": round(loop_ms, 3), "speedup": round(loop_ms / batched_ms, 1),
+            "batched_gbps": round(nbytes / batched_ms / 1e6, 1),
+            "max_rel_err": {"batched": float(ra.max()), "per_segment": float(rb.max())},
+            "median_rel_err": {"batched": float(np.median(ra)), "per_segment": float(np.median(rb))}}
 
     def fp(self, out):
         """n = 2^27 inclusive FP scans: FP32 ~ N(0,1) (seed 4) and BF16 with
