@@ -13,10 +13,14 @@ namespace {
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+// The persistent segmented kernel unless the handle asks for one workgroup
+// per tile.
+bool seg_persistent(const gps_handle* h) { return h->schedule != GPS_SCHEDULE_TILE; }
+
 // Segmented kernels use the handle's tile shape when it is one of theirs,
 // otherwise 1024 threads x 4 uint4 (16384-element tiles).
 gps::TileShape seg_shape(const gps_handle* h) {
-    return gps::seg_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
+    return gps::seg_shape_supported(h->shape, seg_persistent(h)) ? h->shape : gps::TileShape{1024, 4};
 }
 
 int check_seg(gps_handle* h, const void* in, const void* out, const void* offsets, uint64_t n) {
@@ -37,7 +41,7 @@ int enqueue_segmented(gps_handle* h, const uint32_t* in, uint32_t* out, const ui
                       uint64_t num_segments, uint64_t n, bool inclusive, hipStream_t s) {
     if (n == 0) return GPS_OK;
     HIP_TRY(gps::before_epoch_launch(h, s));
-    HIP_TRY(gps::launch_segscan_u32(seg_shape(h), inclusive, in, out, offsets, num_segments, n,
+    HIP_TRY(gps::launch_segscan_u32(seg_shape(h), inclusive, seg_persistent(h), in, out, offsets, num_segments, n,
                                     aligned16(in) && aligned16(out), h->status, h->ticket, h->err, s));
     return GPS_OK;
 }
@@ -125,6 +129,7 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
     *out = nullptr;
     int rc = check_seg(h, d_in, d_out, d_offsets, n);
     if (rc) return rc;
+    HIP_TRY(gps::prepare_segscan_u32(seg_shape(h), inclusive != 0, seg_persistent(h)));
     hipStream_t s = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     gps_graph* g = new (std::nothrow) gps_graph();
@@ -136,9 +141,9 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
     if (e == hipSuccess) {
         // No re-arm check inside a capture: a graph node must not depend on
         // the host-side launch count (gps_reset re-arms explicitly).
-        rc = n ? (gps::launch_segscan_u32(seg_shape(h), inclusive != 0, d_in, d_out, d_offsets, num_segments, n,
-                                          aligned16(d_in) && aligned16(d_out), h->status, h->ticket, h->err, s) ==
-                          hipSuccess
+        rc = n ? (gps::launch_segscan_u32(seg_shape(h), inclusive != 0, seg_persistent(h), d_in, d_out, d_offsets,
+                                          num_segments, n, aligned16(d_in) && aligned16(d_out), h->status,
+                                          h->ticket, h->err, s) == hipSuccess
                       ? GPS_OK
                       : GPS_ERR_HIP)
                : GPS_OK;

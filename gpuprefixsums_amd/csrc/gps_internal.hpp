@@ -43,9 +43,13 @@ hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, h
 hipError_t launch_rts_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
                           uint32_t num_tiles, uint32_t vec_ok, uint32_t* tile_sums, hipStream_t stream);
 
-// Batched segmented scan (one launch, one workgroup per tile).
-bool seg_shape_supported(TileShape s);
-hipError_t launch_segscan_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out,
+// Batched segmented scan in one launch: persistent (grid = resident
+// workgroups, pipelined) or one workgroup per tile.
+bool seg_shape_supported(TileShape s, bool persistent);
+// Runs the persistent kernel's one-time occupancy query (call before a
+// stream capture, so the capture holds only the launch).
+hipError_t prepare_segscan_u32(TileShape s, bool inclusive, bool persistent);
+hipError_t launch_segscan_u32(TileShape s, bool inclusive, bool persistent, const uint32_t* in, uint32_t* out,
                               const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,
                               uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
 

@@ -37,18 +37,19 @@ __device__ __forceinline__ uint64_t find_first_segment(const uint64_t* offsets, 
 }
 
 // Sets bit (o - a) of s_heads for every offset o in [a, b), scanning the
-// offsets from index s0 (the first one >= a) with the whole workgroup; bit 0
-// of tile 0 is always set (position 0 starts a segment). s_heads must be
-// zeroed and s0 visible to all threads (barriers are the caller's).
-template <int BLOCK>
-__device__ __forceinline__ void mark_heads(uint32_t* s_heads, const uint64_t* offsets, uint64_t S, uint64_t s0,
-                                           uint64_t a, uint64_t b, bool first_tile) {
-    if (first_tile && threadIdx.x == 0) atomicOr(&s_heads[0], 1u);
-    for (uint64_t s = s0 + threadIdx.x; s < S; s += 4u * BLOCK) {
+// offsets from index s0 (the first one >= a) with `count` threads (tid in
+// [0, count)); bit 0 of tile 0 is always set (position 0 starts a segment).
+// s_heads must be zeroed and s0 visible to those threads (barriers are the
+// caller's).
+__device__ __forceinline__ void mark_heads_by(uint32_t* s_heads, const uint64_t* offsets, uint64_t S, uint64_t s0,
+                                              uint64_t a, uint64_t b, bool first_tile, uint32_t tid,
+                                              uint32_t count) {
+    if (first_tile && tid == 0) atomicOr(&s_heads[0], 1u);
+    for (uint64_t s = s0 + tid; s < S; s += 4ull * count) {
         uint64_t o[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const uint64_t su = s + (uint64_t)u * BLOCK;
+            const uint64_t su = s + (uint64_t)u * count;
             o[u] = su < S ? offsets[su] : b;
         }
         bool past = false;
@@ -63,6 +64,13 @@ __device__ __forceinline__ void mark_heads(uint32_t* s_heads, const uint64_t* of
         }
         if (past) break;
     }
+}
+
+// mark_heads_by with the whole workgroup.
+template <int BLOCK>
+__device__ __forceinline__ void mark_heads(uint32_t* s_heads, const uint64_t* offsets, uint64_t S, uint64_t s0,
+                                           uint64_t a, uint64_t b, bool first_tile) {
+    mark_heads_by(s_heads, offsets, S, s0, a, b, first_tile, threadIdx.x, BLOCK);
 }
 
 // The 4 head bits of the elements e .. e+3 (e a multiple of 4).

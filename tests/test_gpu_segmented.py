@@ -36,14 +36,21 @@ def layouts():
 LAYOUTS = list(layouts())
 
 
+SCHEDULES = {"persistent": gps.SCHEDULE_PERSISTENT, "tile": gps.SCHEDULE_TILE}
+
+
 @pytest.mark.parametrize("inclusive", [False, True])
-@pytest.mark.parametrize("shape", [(0, 0), (256, 4), (512, 4), (1024, 2), (1024, 4)])
+@pytest.mark.parametrize("schedule", list(SCHEDULES))
+@pytest.mark.parametrize("shape", [(0, 0), (256, 4), (512, 4), (1024, 2), (1024, 4), (1024, 8)])
 @pytest.mark.parametrize("name,off", LAYOUTS, ids=[x[0] for x in LAYOUTS])
-def test_segmented_matches_oracle(name, off, shape, inclusive):
+def test_segmented_matches_oracle(name, off, shape, schedule, inclusive):
+    """Both segmented kernels: the persistent one (default schedule; a 1024x8
+    handle runs it at 1024x4) and one workgroup per tile."""
     n = int(off[-1])
     x = oracle.gen_u32(17 + n, n, 32)
     want = oracle.segmented_scan_u32(x, off, inclusive)
-    with gps.ScanHandle(gps.ALGO_CSDL, max(n, 1), block_threads=shape[0], vec_per_thread=shape[1]) as h:
+    with gps.ScanHandle(gps.ALGO_CSDL, max(n, 1), block_threads=shape[0], vec_per_thread=shape[1],
+                        schedule=SCHEDULES[schedule]) as h:
         d_in, d_off = to_dev(x), to_dev(off)
         d_out = sentinel_u32(n + 16)
         h.segmented(d_in, d_out, d_off, off.size - 1, n, inclusive)
