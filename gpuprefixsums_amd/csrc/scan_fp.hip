@@ -376,11 +376,19 @@ __global__ __launch_bounds__(BLOCK) void segscan_fp(SegFpParams p) {
         t = (nib & 2u) ? ff{x[k].y, 0.f} : ff_add1<COMP>(t, x[k].y);
         t = (nib & 4u) ? ff{x[k].z, 0.f} : ff_add1<COMP>(t, x[k].z);
         t = (nib & 8u) ? ff{x[k].w, 0.f} : ff_add1<COMP>(t, x[k].w);
-        uint32_t sf = nib != 0u;
+        // A vector with no head in any lane (the common case for long
+        // segments) takes the plain pair scan: no flag moves, no selects.
+        const bool any_head = ballot(nib != 0u) != 0ull;
+        uint32_t sf = 0u, ef = 0u;
         ff sv = t;
-        wave_seg_scan_ff<COMP>(sf, sv);
+        if (any_head) {
+            sf = nib != 0u;
+            wave_seg_scan_ff<COMP>(sf, sv);
+            ef = GPS_DPP_I(sf, 0x138, 0xf, true);  // wave_shr:1, lane 0 gets 0
+        } else {
+            sv = wave_incl_scan_ff<COMP>(t);
+        }
         const ff ev = {GPS_DPP_F(sv.s, 0x138, 0xf, true), COMP ? GPS_DPP_F(sv.c, 0x138, 0xf, true) : 0.f};
-        const uint32_t ef = GPS_DPP_I(sf, 0x138, 0xf, true);  // wave_shr:1, lane 0 gets (0, 0)
         lane_carry[k] = ef ? ev : ff_add<COMP>(run, ev);
         open |= (ef | run_f) ? 0u : (1u << k);
         const uint32_t tf = readlane(sf, 63);
