@@ -221,9 +221,6 @@ csdl_pp_u32(ScanParams p) {
 //            of t_i -> publish inclusive, prefix (prefix_seq = i+1)
 // Every LDS wait is bounded by the device watchdog (error bit, then exit).
 // ---------------------------------------------------------------------------
-#ifndef GPS_CW_EARLY_TICKET
-#define GPS_CW_EARLY_TICKET 1
-#endif
 
 // Spins until *addr >= v (workgroup-scope acquire); false if the watchdog fires.
 __device__ __forceinline__ bool lds_wait_ge(const uint32_t* addr, uint32_t v, uint32_t* err) {
@@ -275,17 +272,16 @@ csdl_cw_u32(ScanParams p) {
         tile = readlane(tile, 0);
         epoch = readlane(epoch, 0);
         for (uint32_t i = 0; tile < p.num_tiles; ++i) {
+            // the next ticket is taken before this tile's waves arrive, so its
+            // round trip overlaps their scan
             uint64_t old = 0;
-            if (GPS_CW_EARLY_TICKET && lane == 0)
-                old = __hip_atomic_fetch_add(p.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) old = __hip_atomic_fetch_add(p.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bool ok = true;
             if (lane < (uint32_t)DW) ok = lds_wait_ge(&s_arrived[lane], i + 1u, p.err);
             if (ballot(!ok)) return;
             const uint32_t total = wave_sum(lane < (uint32_t)DW ? s_tot[i & 1u][lane] : 0u);
             if (lane == 0) {
                 atomic_store_agent(p.status + tile, make_status(total, tile ? kFlagAgg : kFlagIncl, epoch));
-                if (!GPS_CW_EARLY_TICKET)
-                    old = __hip_atomic_fetch_add(p.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const uint32_t next = (uint32_t)old;
                 if (next == total_tickets - 1u)
                     atomic_store_agent(p.ticket, (uint64_t)next_epoch((uint32_t)(old >> 32)) << 32);

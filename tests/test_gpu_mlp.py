@@ -77,3 +77,23 @@ def test_trained_tuner_regret_on_held_out_workloads():
     assert r_mlp["mean"] <= 0.05
     assert r_mlp["mean"] < r_def["mean"]
     assert r_mlp["mean"] < r_stat["mean"]
+
+
+def test_tuner_choose_with_the_model_and_by_exhaustive_search():
+    """Tuner.choose: the shipped model's pick, and the exhaustive fallback
+    (no model) that times every candidate on the GPU and keeps the fastest."""
+    from gpuprefixsums_amd.tuner import DEFAULT_MODEL, Tuner
+    from gpuprefixsums_amd.tuner.space import Workload, candidates, lognormal_partition
+    rng = np.random.default_rng(3)
+    w_u32 = Workload(0, 100_000, "u32", 12, lognormal_partition(100_000, 12, rng))
+    w_f32 = Workload(1, 50_000, "f32", 3, lognormal_partition(50_000, 3, rng))
+    if DEFAULT_MODEL.exists():
+        for w in (w_u32, w_f32):
+            assert Tuner().choose(w) in candidates(w.dtype)
+    exhaustive = Tuner(model_path=None)
+    for w in (w_u32, w_f32):
+        c = exhaustive.choose(w)
+        assert c in candidates(w.dtype)
+        times = exhaustive._runner.time_all(w)
+        # the pick is among the fastest (timings repeat within noise)
+        assert times[c.name] <= 1.5 * min(times.values())
