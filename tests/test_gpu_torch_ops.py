@@ -1,0 +1,74 @@
+"""torch.ops.gps.cumsum / segmented_cumsum on the GPU against the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import gpuprefixsums_amd.torch_ops  # noqa: E402,F401
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a ROCm device")
+
+
+def u32_tensor(x: np.ndarray, dtype):
+    t = torch.from_numpy(x.view(np.int32)).cuda()
+    return t.view(torch.uint32) if dtype == torch.uint32 else t
+
+
+def u32_numpy(t: torch.Tensor) -> np.ndarray:
+    return (t.view(torch.int32) if t.dtype == torch.uint32 else t).cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("dtype", [torch.uint32, torch.int32])
+@pytest.mark.parametrize("inclusive", [True, False])
+@pytest.mark.parametrize("n", [1, 1000, (1 << 21) + 3])
+def test_integer_cumsum_is_bit_exact(n, inclusive, dtype):
+    x = oracle.gen_u32(n, n, 32)
+    y = torch.ops.gps.cumsum(u32_tensor(x, dtype), inclusive=inclusive)
+    assert y.dtype == dtype
+    assert np.array_equal(u32_numpy(y), oracle.scan_u32(x, inclusive))
+
+
+def test_float_cumsum_within_one_rounding():
+    x = oracle.gen_f32_normal(4, (1 << 20) + 1)
+    y = torch.ops.gps.cumsum(torch.from_numpy(x).cuda()).cpu().numpy()
+    y64 = oracle.scan_f32_f64(x)
+    S = oracle.abs_scan_f32_f64(x)
+    assert np.all(np.abs(y - y64) <= 2.0 ** -23 * np.abs(y64) + 2.0 ** -36 * S)
+    b = oracle.gen_bf16_logmag(5, 70001)
+    yb = torch.ops.gps.cumsum(torch.from_numpy(b.view(np.int16)).cuda().view(torch.bfloat16))
+    assert yb.dtype == torch.float32
+    y64 = oracle.scan_bf16_f64(b)
+    assert np.all(np.abs(yb.cpu().numpy() - y64) <= 2.0 ** -23 * np.abs(y64) + 2.0 ** -36 * oracle.abs_scan_bf16_f64(b))
+
+
+def test_segmented_cumsum_integer_and_float():
+    off = oracle.gen_lognormal_offsets(9, 300, median=500.0, sigma=1.5, lo=1, hi=1 << 16)
+    n = int(off[-1])
+    d_off = torch.from_numpy(off.view(np.int64)).cuda()
+    x = oracle.gen_u32(3, n, 32)
+    y = torch.ops.gps.segmented_cumsum(u32_tensor(x, torch.uint32), d_off)
+    assert np.array_equal(u32_numpy(y), oracle.segmented_scan_u32(x, off, False))
+    xf = oracle.gen_f32_normal(6, n)
+    yf = torch.ops.gps.segmented_cumsum(torch.from_numpy(xf).cuda(), d_off, inclusive=True).cpu().numpy()
+    y64 = oracle.segmented_scan_fp_f64(xf, off)
+    S = oracle.segmented_scan_fp_f64(xf, off, absolute=True)
+    assert np.all(np.abs(yf - y64) <= 2.0 ** -23 * np.abs(y64) + 2.0 ** -36 * S)
+
+
+def test_runs_on_the_current_stream_and_grows_its_handle():
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for n in (1 << 10, 1 << 22, (1 << 23) + 5):  # the cached handle grows past 2^20
+            x = oracle.gen_u32(n + 1, n, 32)
+            y = torch.ops.gps.cumsum(u32_tensor(x, torch.int32))
+            side.synchronize()
+            assert np.array_equal(u32_numpy(y), oracle.scan_u32(x, True))
