@@ -35,7 +35,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
 ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
-            "tuner"]
+            "struct", "tuner"]
 
 
 def log(msg: str) -> None:
@@ -266,6 +266,33 @@ class Legs:
             if h.device_error():
                 raise RuntimeError("device error in fp leg")
         out["fp_n2p27"] = res
+
+    def struct(self, out):
+        """Member-wise scan of n/4 = 2^26 four-member uint32 structs over the
+        same 1 GiB input (inclusive and exclusive), checked member by member
+        against the oracle; 32 B per struct."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        ns = self.n // 4
+        st = torch.cuda.current_stream()
+        res = {}
+        with gps.ScanHandle(gps.ALGO_CSDL, ns) as h:
+            for inclusive in (True, False):
+                for _ in range(3):
+                    h.scan_u32x4(self.d_in, self.d_out, ns, inclusive)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                for _ in range(20):
+                    h.scan_u32x4(self.d_in, self.d_out, ns, inclusive)
+                e1.record(st)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / 20
+                want = oracle.scan_u32x4(self.d_in.cpu().numpy().view(np.uint32), inclusive)
+                got = self.d_out.cpu().numpy().view(np.uint32).reshape(-1, 4)
+                if not np.array_equal(got, want) or h.device_error():
+                    raise RuntimeError("struct scan differs from the oracle")
+                res["inclusive" if inclusive else "exclusive"] = {"gbps": round(gbps(ns, ms, 32), 1),
+                                                                  "kernel_ms": round(ms, 5)}
+        out["struct_u32x4_n2p26"] = res
 
     def tuner(self, out):
         """Learned tuner: regret of the MLP cost model's choices (inference on

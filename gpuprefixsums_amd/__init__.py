@@ -118,6 +118,11 @@ class ScanHandle:
                                        _stream(stream), iters, ms))
         return list(ms)
 
+    def scan_u32x4(self, d_in, d_out, n: int, inclusive: bool = True, stream=None) -> None:
+        """Member-wise scan of n 4-member uint32 structs (uint4 elements)."""
+        fn = load().gps_inclusive_scan_u32x4 if inclusive else load().gps_exclusive_scan_u32x4
+        check(fn(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
+
     def inclusive_f32(self, d_in, d_out, n: Optional[int] = None, compensated: bool = True, stream=None) -> None:
         """FP32 -> FP32 inclusive scan (two-sum compensated carries unless compensated=False)."""
         n = _numel(d_in) if n is None else n

@@ -32,6 +32,7 @@ LIB_SOURCES = [
     "scan_fp.hip",
     "mlp.hip",
     "vendor_rocprim.hip",
+    "scan_u32x4.hip",
     "capi.cpp",
     "capi_ext.cpp",
     "dispatchers.cpp",

@@ -169,7 +169,10 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     if (stall_period == 1) return fail(GPS_ERR_INVALID, "stall_period must be 0 or >= 2");
     if (stall_period && algo != GPS_ALGO_CSDLDF)
         return fail(GPS_ERR_INVALID, "stall emulation needs the decoupled-fallback algorithm");
-    const uint64_t tiles = (max_n + kMinTileElems - 1) / kMinTileElems;
+    // One word per kMinTileElems elements, plus three so that the struct scan
+    // (four words per tile of >= 4096 structs) fits for every n <= max_n:
+    // 4 * ceil(n / 4096) <= ceil(n / 1024) + 3.
+    const uint64_t tiles = (max_n + kMinTileElems - 1) / kMinTileElems + 3;
     if (tiles >= (1ull << 31)) return fail(GPS_ERR_TOO_LARGE, "max_n too large");
 
     gps_handle* h = new gps_handle();

@@ -90,6 +90,25 @@ int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offset
     return GPS_OK;
 }
 
+int scan_u32x4(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, void* stream) {
+    if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "struct scans need a CSDL handle");
+    if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
+    if (n > h->max_n)
+        return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
+                    (unsigned long long)h->max_n);
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u)
+        return fail(GPS_ERR_INVALID, "uint4 struct buffers must be 16-byte aligned");
+    if (n == 0) return GPS_OK;
+    const gps::TileShape s = gps::v4_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
+    const uint64_t tile = (uint64_t)s.block * s.vec;
+    if (4 * ((n + tile - 1) / tile) > h->max_tiles) return fail(GPS_ERR_TOO_LARGE, "not enough status words");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(gps::before_epoch_launch(h, st));
+    HIP_TRY(gps::launch_csdl_u32x4(s, inclusive, in, out, n, h->status, h->ticket, h->err, st));
+    return GPS_OK;
+}
+
 }  // namespace
 
 struct gps_graph {
@@ -182,6 +201,14 @@ int gps_segmented_scan_f32(gps_handle* h, const float* d_in, float* d_out, const
 int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, const uint64_t* d_offsets,
                                 uint64_t num_segments, uint64_t n, int compensated, void* stream) {
     return segscan_fp(h, d_in, d_out, d_offsets, num_segments, n, true, compensated, stream);
+}
+
+int gps_inclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream) {
+    return scan_u32x4(h, d_in, d_out, n, true, stream);
+}
+
+int gps_exclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream) {
+    return scan_u32x4(h, d_in, d_out, n, false, stream);
 }
 
 int gps_inclusive_scan_fp_per_segment(gps_handle* h, const void* d_in, float* d_out, const uint64_t* h_offsets,

@@ -62,6 +62,11 @@ hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const voi
                              const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,
                              uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
 
+// Member-wise scan of n 4-member uint32 structs (4 status words per tile).
+bool v4_shape_supported(TileShape s);
+hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
+                             uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
+
 // Clears `count` status words and arms the ticket word at epoch 1.
 hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream);
 

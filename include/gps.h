@@ -117,6 +117,18 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
 int gps_graph_launch(gps_graph* g, void* stream);
 int gps_graph_destroy(gps_graph* g);
 
+/* ---- member-wise scan of 4-member uint32 structs ---------------------------
+ * n structs of four uint32 members each (uint4 elements, 16-byte aligned
+ * buffers, d_in == d_out allowed): for each member m independently,
+ * out[i].m = sum of in[j].m over j <= i (inclusive) or j < i (exclusive),
+ * mod 2^32. One launch; a status word per member per tile and a lookback that
+ * takes a tile when all four agree. Replaces the reference's struct scan
+ * (GPUPrefixSumsWebGPUapis/SharedShaders/csdldf_struct.wgsl:60-300, a vec4
+ * scanned with per-member status words). Needs a GPS_ALGO_CSDL handle with
+ * max_n >= n. */
+int gps_inclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
+int gps_exclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
+
 /* ---- floating-point inclusive scans (no reference counterpart) ----------- */
 
 /* FP32 or BF16 (raw uint16 bit patterns) input, FP32 output, accumulated in

@@ -128,6 +128,19 @@ def scan_u32_mt(x, inclusive: bool = True, threads: int = 0) -> np.ndarray:
     return out
 
 
+def scan_u32x4(x, inclusive: bool = True) -> np.ndarray:
+    """Member-wise scan of 4-member uint32 structs: x is (n, 4) or 4n
+    interleaved uint32; each member column is scanned on its own, mod 2^32
+    (GPUPrefixSumsWebGPUapis/SharedShaders/csdldf_struct.wgsl:153-291 keeps
+    one reduction per member per tile; validate.wgsl:51-60 checks all-ones
+    input against (i >> 2) + 1 per uint32)."""
+    x = _c(x, np.uint32).reshape(-1, 4)
+    out = np.empty_like(x)
+    for m in range(4):
+        out[:, m] = scan_u32(x[:, m], inclusive)
+    return out
+
+
 def segmented_scan_u32(x, offsets, inclusive: bool = False) -> np.ndarray:
     x, offsets = _c(x, np.uint32), _c(offsets, np.uint64)
     assert offsets.size >= 1 and int(offsets[-1]) == x.size
