@@ -36,7 +36,7 @@ int hip_fail(hipError_t e, const char* what) {
 
 hipError_t before_epoch_launch(gps_handle* h, hipStream_t s) {
     if (h->launches >= kLaunchesBeforeReset) {
-        const hipError_t e = launch_reset_status(h->status, h->max_tiles, h->ticket, s);
+        const hipError_t e = launch_reset_status(h->status, (uint64_t)h->max_tiles * kSlotWords, h->ticket, s);
         if (e != hipSuccess) return e;
         h->launches = 0;
     }
@@ -169,10 +169,10 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     if (stall_period == 1) return fail(GPS_ERR_INVALID, "stall_period must be 0 or >= 2");
     if (stall_period && algo != GPS_ALGO_CSDLDF)
         return fail(GPS_ERR_INVALID, "stall emulation needs the decoupled-fallback algorithm");
-    // One word per kMinTileElems elements, plus three so that the struct scan
-    // (four words per tile of >= 4096 structs) fits for every n <= max_n:
-    // 4 * ceil(n / 4096) <= ceil(n / 1024) + 3.
-    const uint64_t tiles = (max_n + kMinTileElems - 1) / kMinTileElems + 3;
+    // One status slot (kSlotWords words) per kMinTileElems elements: every
+    // kernel's tile holds at least that many elements (structs for the struct
+    // scan), so a launch over n <= max_n never needs more slots.
+    const uint64_t tiles = (max_n + kMinTileElems - 1) / kMinTileElems;
     if (tiles >= (1ull << 31)) return fail(GPS_ERR_TOO_LARGE, "max_n too large");
 
     gps_handle* h = new gps_handle();
@@ -184,12 +184,12 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     h->schedule = schedule;
     h->max_tiles = (uint32_t)tiles;
     hipError_t e = hipGetDevice(&h->device);
-    if (e == hipSuccess) e = hipMalloc(&h->status, tiles * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&h->status, tiles * gps::kSlotWords * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&h->ticket, sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&h->counters, 4 * sizeof(unsigned long long));
     if (e == hipSuccess && algo == GPS_ALGO_RTS) e = hipMalloc(&h->tile_sums, tiles * sizeof(uint32_t));
-    if (e == hipSuccess) e = gps::launch_reset_status(h->status, tiles, h->ticket, nullptr);
+    if (e == hipSuccess) e = gps::launch_reset_status(h->status, tiles * gps::kSlotWords, h->ticket, nullptr);
     if (e == hipSuccess) e = hipMemset(h->err, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(h->counters, 0, 4 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -254,7 +254,8 @@ int gps_device_error(gps_handle* h, uint32_t* bits) {
 
 int gps_reset(gps_handle* h, void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
-    HIP_TRY(gps::launch_reset_status(h->status, h->max_tiles, h->ticket, static_cast<hipStream_t>(stream)));
+    HIP_TRY(gps::launch_reset_status(h->status, (uint64_t)h->max_tiles * gps::kSlotWords, h->ticket,
+                                     static_cast<hipStream_t>(stream)));
     h->launches = 0;
     return GPS_OK;
 }

@@ -7,6 +7,12 @@
 
 namespace gps {
 
+// Tile status words are laid out one tile per 128-byte slot of kSlotWords
+// words (a tile with several words keeps them all in its slot), so the
+// publications of neighbouring tiles, made from different XCDs, never share a
+// cache line. A handle holds one slot per kMinTileElems elements of max_n.
+constexpr uint64_t kSlotWords = 16;
+
 // Launch parameters shared by the single-pass uint32 kernels (CSDL, CSDLDF).
 struct ScanParams {
     const uint32_t* in;
@@ -64,8 +70,10 @@ hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const voi
 
 // Member-wise scan of n 4-member uint32 structs (4 status words per tile).
 bool v4_shape_supported(TileShape s);
-hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
-                             uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
+// persistent: resident workgroups with the pipelined ticket loop; otherwise
+// one workgroup per tile.
+hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, bool persistent, const uint32_t* in, uint32_t* out,
+                             uint64_t n, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
 
 // Clears `count` status words and arms the ticket word at epoch 1.
 hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream);

@@ -20,7 +20,7 @@ struct gps_handle {
     uint32_t spin_limit = 4;
     uint32_t stall_period = 0;
     uint32_t schedule = GPS_SCHEDULE_TILE;
-    uint32_t max_tiles = 0;     // status words allocated (for kMinTileElems tiles)
+    uint32_t max_tiles = 0;     // status slots allocated (one per kMinTileElems elements)
     int device = 0;
     uint64_t* status = nullptr;
     uint64_t* ticket = nullptr;

@@ -76,13 +76,13 @@ __device__ __forceinline__ float readlane_f(float x, int lane) {
 
 // Status pair for the FP scans: two 64-bit words per tile, both carrying the
 // flag and epoch (same layout as the integer words, value = float bits):
-// status[2t] = s, status[2t+1] = c. Readers accept a tile only when both words
+// word 0 of tile t's slot = s, word 1 = c. Readers accept a tile only when both words
 // are current and agree on the flag; each flag is written once per launch, so
 // agreeing words belong to the same publication.
 template <bool COMP>
 __device__ __forceinline__ void publish_ff(uint64_t* status, uint32_t tile, ff v, uint64_t flag, uint32_t epoch) {
-    atomic_store_agent(status + 2ull * tile, make_status(__builtin_bit_cast(uint32_t, v.s), flag, epoch));
-    atomic_store_agent(status + 2ull * tile + 1, make_status(COMP ? __builtin_bit_cast(uint32_t, v.c) : 0u, flag, epoch));
+    atomic_store_agent(status + kSlotWords * tile, make_status(__builtin_bit_cast(uint32_t, v.s), flag, epoch));
+    atomic_store_agent(status + kSlotWords * tile + 1, make_status(COMP ? __builtin_bit_cast(uint32_t, v.c) : 0u, flag, epoch));
 }
 
 // Window lookback over pair words (one full wave); returns the exclusive
@@ -98,8 +98,8 @@ __device__ __forceinline__ ff lookback_ff(const uint64_t* status, uint32_t tile,
         // the sentinel, so no divergent region precedes the ballots.
         const int64_t idx = top - (int64_t)lane;
         const int64_t at = idx >= 0 ? idx : 0;
-        const uint64_t ls = atomic_load_agent(status + 2 * at);
-        const uint64_t lc = atomic_load_agent(status + 2 * at + 1);
+        const uint64_t ls = atomic_load_agent(status + kSlotWords * at);
+        const uint64_t lc = atomic_load_agent(status + kSlotWords * at + 1);
         const uint64_t sentinel = make_status(0u, kFlagIncl, epoch);
         const uint64_t ws = idx >= 0 ? ls : sentinel, wc = idx >= 0 ? lc : sentinel;
         const uint32_t fs = status_flag(ws, epoch), fc = status_flag(wc, epoch);

@@ -90,10 +90,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES
                 s_prefix = 0;
             }
         } else {
-            if (lane == 0) atomic_store_agent(p.status + f.tile, make_status(f.tile_total, kFlagAgg, f.epoch));
+            if (lane == 0) atomic_store_agent(p.status + kSlotWords * f.tile, make_status(f.tile_total, kFlagAgg, f.epoch));
             const uint32_t prefix = lookback_u32(p.status, f.tile, f.epoch, p.err);
             if (lane == 0) {
-                atomic_store_agent(p.status + f.tile, make_status(prefix + f.tile_total, kFlagIncl, f.epoch));
+                atomic_store_agent(p.status + kSlotWords * f.tile, make_status(prefix + f.tile_total, kFlagIncl, f.epoch));
                 s_prefix = prefix;
             }
         }
@@ -160,7 +160,7 @@ csdl_pp_u32(ScanParams p) {
             // follows publication order, so lookbacks find contiguous ready
             // runs. (Taking the ticket before the scan, or a whole tile
             // ahead, measured 8% slower at 2^28.)
-            atomic_store_agent(p.status + tile, make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+            atomic_store_agent(p.status + kSlotWords * tile, make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
             s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
         }
         __syncthreads();
@@ -184,7 +184,7 @@ csdl_pp_u32(ScanParams p) {
             if (tile != 0) {
                 prefix = lb.finish(p.status, epoch, p.err, true);
                 if (lane == 0)
-                    atomic_store_agent(p.status + tile, make_status(prefix + tile_total, kFlagIncl, epoch));
+                    atomic_store_agent(p.status + kSlotWords * tile, make_status(prefix + tile_total, kFlagIncl, epoch));
             }
             if (lane == 0) s_prefix = prefix;
         }
@@ -281,7 +281,7 @@ csdl_cw_u32(ScanParams p) {
             if (ballot(!ok)) return;
             const uint32_t total = wave_sum(lane < (uint32_t)DW ? s_tot[i & 1u][lane] : 0u);
             if (lane == 0) {
-                atomic_store_agent(p.status + tile, make_status(total, tile ? kFlagAgg : kFlagIncl, epoch));
+                atomic_store_agent(p.status + kSlotWords * tile, make_status(total, tile ? kFlagAgg : kFlagIncl, epoch));
                 const uint32_t next = (uint32_t)old;
                 if (next == total_tickets - 1u)
                     atomic_store_agent(p.ticket, (uint64_t)next_epoch((uint32_t)(old >> 32)) << 32);
@@ -291,7 +291,7 @@ csdl_cw_u32(ScanParams p) {
             uint32_t prefix = 0;
             if (tile != 0) {
                 prefix = lookback_u32(p.status, tile, epoch, p.err);
-                if (lane == 0) atomic_store_agent(p.status + tile, make_status(prefix + total, kFlagIncl, epoch));
+                if (lane == 0) atomic_store_agent(p.status + kSlotWords * tile, make_status(prefix + total, kFlagIncl, epoch));
             }
             if (lane == 0) {
                 s_prefix[i & 1u] = prefix;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
         unsigned long long stall_timeouts = 0;
         if (p.stall_period && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
             uint32_t polls = 0;
-            while (status_flag(atomic_load_agent(p.status + tile), epoch) == 0u) {
+            while (status_flag(atomic_load_agent(p.status + kSlotWords * tile), epoch) == 0u) {
                 if (watchdog_expired(polls, p.err)) {
                     stall_timeouts = 1;
                     raise_device_error(p.err, kErrStallWatchdog);
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        atomic_xchg_agent(p.status + tile, make_status(f.tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+        atomic_xchg_agent(p.status + kSlotWords * tile, make_status(f.tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
         if (p.counters) atomic_add_counter(p.counters + 3, stall_timeouts);
         if (tile == 0) s_prefix = 0;
     }
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                 while (true) {
                     const int64_t idx = top - (int64_t)lane;
                     // branch-free window read (no divergent region before the ballots)
-                    const uint64_t v = atomic_load_agent(p.status + (idx >= 0 ? idx : 0));
+                    const uint64_t v = atomic_load_agent(p.status + kSlotWords * (uint64_t)(idx >= 0 ? idx : 0));
                     const uint64_t w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
                     const uint32_t fl = status_flag(w, epoch);
                     const uint64_t ready = ballot(fl != 0u);
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                 }
                 if (lane == 0) {
                     if (done) {
-                        atomic_xchg_agent(p.status + tile, make_status(prefix + f.tile_total, kFlagIncl, epoch));
+                        atomic_xchg_agent(p.status + kSlotWords * tile, make_status(prefix + f.tile_total, kFlagIncl, epoch));
                         s_prefix = prefix;
                     } else {
                         s_fb_tile = (uint32_t)top;
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                 for (uint32_t q = 0; q < C::kWaves; ++q) fsum += s_red[q];
                 const uint64_t mine = make_status(fsum, j ? kFlagAgg : kFlagIncl, epoch);
                 uint64_t found = mine;
-                if (lane == 0) found = atomic_cas_agent(p.status + j, stale, mine);
+                if (lane == 0) found = atomic_cas_agent(p.status + kSlotWords * j, stale, mine);
                 found = ((uint64_t)readlane((uint32_t)(found >> 32), 0) << 32) | readlane((uint32_t)found, 0);
                 ++c_fallbacks;
                 uint32_t fl;
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                 spins = 0;
                 if (lane == 0) {
                     if (done) {
-                        atomic_xchg_agent(p.status + tile, make_status(prefix + f.tile_total, kFlagIncl, epoch));
+                        atomic_xchg_agent(p.status + kSlotWords * tile, make_status(prefix + f.tile_total, kFlagIncl, epoch));
                         s_prefix = prefix;
                     }
                     s_cas_done = done;

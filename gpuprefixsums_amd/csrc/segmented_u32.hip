@@ -180,15 +180,15 @@ __global__ __launch_bounds__(BLOCK) void segscan_u32(SegParams p) {
     const bool starts_with_head = (s_heads[0] & 1u) != 0u;
     if (wave == 0) {
         if (tile_f) {
-            if (lane == 0) atomic_store_agent(p.status + tile, make_status(tile_v, kFlagIncl, epoch));
+            if (lane == 0) atomic_store_agent(p.status + kSlotWords * tile, make_status(tile_v, kFlagIncl, epoch));
         } else if (lane == 0) {
-            atomic_store_agent(p.status + tile, make_status(tile_v, tile ? kFlagAgg : kFlagIncl, epoch));
+            atomic_store_agent(p.status + kSlotWords * tile, make_status(tile_v, tile ? kFlagAgg : kFlagIncl, epoch));
         }
         uint32_t prefix = 0;
         if (tile != 0 && !starts_with_head) {
             prefix = lookback_u32(p.status, tile, epoch, p.err);
             if (!tile_f && lane == 0)
-                atomic_store_agent(p.status + tile, make_status(prefix + tile_v, kFlagIncl, epoch));
+                atomic_store_agent(p.status + kSlotWords * tile, make_status(prefix + tile_v, kFlagIncl, epoch));
         }
         if (lane == 0) s_prefix = prefix;
     }
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const bool need_lookback = tile != 0 && (s_heads[cur][0] & 1u) == 0u;
         if (threadIdx.x == 0) {
             // a tile with a head knows its carry-out: publish it inclusive now
-            atomic_store_agent(p.status + tile,
+            atomic_store_agent(p.status + kSlotWords * tile,
                                make_status(tile_v, (tile_f || tile == 0) ? kFlagIncl : kFlagAgg, epoch));
             s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
         }
@@ -310,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             if (need_lookback) {
                 prefix = lb.finish(p.status, epoch, p.err, true);
                 if (!tile_f && lane == 0)
-                    atomic_store_agent(p.status + tile, make_status(prefix + tile_v, kFlagIncl, epoch));
+                    atomic_store_agent(p.status + kSlotWords * tile, make_status(prefix + tile_v, kFlagIncl, epoch));
             }
             if (lane == 0) s_prefix = prefix;
         }

@@ -21,6 +21,7 @@ import gpuprefixsums_amd as gps  # noqa: E402
 from tests.gpu_util import first_mismatch, from_dev_u32, sentinel_u32, sync, to_dev  # noqa: E402
 
 SHAPES = [(512, 8), (1024, 4), (1024, 8)]
+SCHEDULES = {"tile": gps.SCHEDULE_TILE, "persistent": gps.SCHEDULE_PERSISTENT}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -65,30 +66,35 @@ def sizes(tile: int):
     return sorted(s)
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
-def test_all_ones_known_answer(shape):
+def test_all_ones_known_answer(shape, schedule):
     block, vec = shape
     tile = block * vec
-    with gps.ScanHandle(gps.ALGO_CSDL, 4 * tile + 8, block_threads=block, vec_per_thread=vec) as h:
+    with gps.ScanHandle(gps.ALGO_CSDL, 4 * tile + 8, block_threads=block, vec_per_thread=vec,
+                        schedule=SCHEDULES[schedule]) as h:
         for n in sizes(tile):
             got = run(h, np.ones((n, 4), np.uint32), True).reshape(-1)
             want = (np.arange(4 * n, dtype=np.uint32) >> 2) + 1
             assert np.array_equal(got, want), f"{shape} n={n}: {first_mismatch(got, want)}"
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("inclusive", [True, False], ids=["incl", "excl"])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
-def test_random_members_full_range(shape, inclusive):
+def test_random_members_full_range(shape, inclusive, schedule):
     block, vec = shape
     tile = block * vec
-    with gps.ScanHandle(gps.ALGO_CSDL, 4 * tile + 8, block_threads=block, vec_per_thread=vec) as h:
+    with gps.ScanHandle(gps.ALGO_CSDL, 4 * tile + 8, block_threads=block, vec_per_thread=vec,
+                        schedule=SCHEDULES[schedule]) as h:
         for n in sizes(tile):
             check(h, structs(1000 + n, n), inclusive, f"{shape}")
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("inclusive", [True, False], ids=["incl", "excl"])
-def test_in_place(inclusive):
-    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 20) as h:
+def test_in_place(inclusive, schedule):
+    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 20, schedule=SCHEDULES[schedule]) as h:
         for n in (1, 4095, 4096, 40_000):
             check(h, structs(5 + n, n), inclusive, "in place", in_place=True)
 
@@ -116,15 +122,20 @@ def test_smallest_handle_admits_every_n():
                 check(h, structs(max_n + n, n), True, f"max_n={max_n}")
 
 
-def test_repeated_launches_reuse_status_words():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_repeated_launches_reuse_status_words(schedule):
+    # Alternating the plain and the struct scan on one handle: each launch's
+    # epoch makes the other kernel's status words read as not ready.
     n = 1 << 18
     x = structs(3, n)
     want = oracle.scan_u32x4(x, True)
     d_in = to_dev(x.reshape(-1))
     d_out = sentinel_u32(4 * n)
-    with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+    d_tmp = sentinel_u32(4 * n)
+    with gps.ScanHandle(gps.ALGO_CSDL, 4 * n, schedule=SCHEDULES[schedule]) as h:
         for _ in range(20):
             h.scan_u32x4(d_in, d_out, n, True)
+            h.inclusive(d_in, d_tmp, 4 * n)
         sync()
         got = from_dev_u32(d_out).reshape(-1, 4)
         assert np.array_equal(got, want), first_mismatch(got, want)
@@ -134,14 +145,16 @@ def test_repeated_launches_reuse_status_words():
 def test_large_2p25_structs():
     n = 1 << 25  # 512 MiB in, 512 MiB out
     x = structs(25, n)
-    with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
-        d_in = to_dev(x.reshape(-1))
-        d_out = torch.empty(4 * n, dtype=torch.int32, device="cuda")
-        h.scan_u32x4(d_in, d_out, n, True)
-        sync()
-        got = from_dev_u32(d_out).reshape(-1, 4)
     want = oracle.scan_u32x4(x, True)
-    assert np.array_equal(got, want), first_mismatch(got, want)
+    d_in = to_dev(x.reshape(-1))
+    d_out = torch.empty(4 * n, dtype=torch.int32, device="cuda")
+    for name, schedule in SCHEDULES.items():
+        with gps.ScanHandle(gps.ALGO_CSDL, n, schedule=schedule) as h:
+            h.scan_u32x4(d_in, d_out, n, True)
+            sync()
+            got = from_dev_u32(d_out).reshape(-1, 4)
+            assert np.array_equal(got, want), f"{name}: {first_mismatch(got, want)}"
+            assert h.device_error() == 0
 
 
 def test_n_zero_is_a_no_op():
