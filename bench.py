@@ -35,7 +35,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
 ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
-            "struct", "tuner"]
+            "struct", "occupancy", "tuner"]
 
 
 def log(msg: str) -> None:
@@ -293,6 +293,25 @@ class Legs:
                 res["inclusive" if inclusive else "exclusive"] = {"gbps": round(gbps(ns, ms, 32), 1),
                                                                   "kernel_ms": round(ms, 5)}
         out["struct_u32x4_n2p26"] = res
+
+    def occupancy(self, out):
+        """Occupancy estimate (one workgroup launched per tile; those that took a
+        tile are counted) for the persistent u32 scan at 2^28 and the struct
+        scan at 2^26 structs, against the resident grid the normal launch uses;
+        the oversubscribed scans are checked like the others."""
+        gps, np = self.gps, self.np
+        res = {}
+        with gps.ScanHandle(gps.ALGO_CSDL, self.n) as h:
+            m, r = h.estimate_occupancy(self.d_in, self.d_out, self.n)
+            if gps.validate_scan_u32(self.d_in, self.d_out, self.n, True) or h.device_error():
+                raise RuntimeError("oversubscribed scan failed validation")
+            res["u32_n2p28"] = {"workgroups_that_took_a_tile": m, "resident_grid": r, "tiles": -(-self.n // h.tile)}
+            m, r = h.estimate_occupancy(self.d_in, self.d_out, self.n // 4, struct_scan=True)
+            if h.device_error():
+                raise RuntimeError("device error in the struct occupancy run")
+            res["struct_n2p26"] = {"workgroups_that_took_a_tile": m, "resident_grid": r,
+                                   "tiles": -(-(self.n // 4) // 8192)}
+        out["occupancy"] = res
 
     def tuner(self, out):
         """Learned tuner: regret of the MLP cost model's choices (inference on

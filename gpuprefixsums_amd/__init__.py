@@ -123,6 +123,15 @@ class ScanHandle:
         fn = load().gps_inclusive_scan_u32x4 if inclusive else load().gps_exclusive_scan_u32x4
         check(fn(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
 
+    def estimate_occupancy(self, d_in, d_out, n: int, struct_scan: bool = False, stream=None) -> tuple:
+        """Runs the persistent inclusive scan with one workgroup per tile and
+        returns (workgroups that took a tile, resident workgroups of the normal
+        persistent launch)."""
+        measured, resident = ctypes.c_uint32(), ctypes.c_uint32()
+        check(load().gps_estimate_occupancy(self.handle, _ptr(d_in), _ptr(d_out), n, int(struct_scan),
+                                            _stream(stream), ctypes.byref(measured), ctypes.byref(resident)))
+        return measured.value, resident.value
+
     def inclusive_f32(self, d_in, d_out, n: Optional[int] = None, compensated: bool = True, stream=None) -> None:
         """FP32 -> FP32 inclusive scan (two-sum compensated carries unless compensated=False)."""
         n = _numel(d_in) if n is None else n

@@ -187,11 +187,11 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     if (e == hipSuccess) e = hipMalloc(&h->status, tiles * gps::kSlotWords * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&h->ticket, sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&h->counters, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&h->counters, gps::kCounterWords * sizeof(unsigned long long));
     if (e == hipSuccess && algo == GPS_ALGO_RTS) e = hipMalloc(&h->tile_sums, tiles * sizeof(uint32_t));
     if (e == hipSuccess) e = gps::launch_reset_status(h->status, tiles * gps::kSlotWords, h->ticket, nullptr);
     if (e == hipSuccess) e = hipMemset(h->err, 0, sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(h->counters, 0, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(h->counters, 0, gps::kCounterWords * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_handle(h);

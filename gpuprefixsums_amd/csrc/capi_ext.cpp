@@ -204,6 +204,49 @@ int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_ou
     return segscan_fp(h, d_in, d_out, d_offsets, num_segments, n, true, compensated, stream);
 }
 
+int gps_estimate_occupancy(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int struct_scan,
+                           void* stream, uint32_t* measured, uint32_t* resident) {
+    if (!h || !measured || !resident) return fail(GPS_ERR_INVALID, "null argument");
+    if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "the occupancy estimate needs a CSDL handle");
+    if (!d_in || !d_out) return fail(GPS_ERR_INVALID, "null data pointer");
+    if (n == 0) return fail(GPS_ERR_INVALID, "n must be positive");
+    if (n > h->max_n)
+        return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
+                    (unsigned long long)h->max_n);
+    const uintptr_t mask = struct_scan ? 15u : 3u;
+    if ((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) & mask)
+        return fail(GPS_ERR_INVALID, struct_scan ? "uint4 struct buffers must be 16-byte aligned"
+                                                 : "uint32 buffers must be 4-byte aligned");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned long long* count = h->counters + gps::kCounterOccupancy;
+    HIP_TRY(hipMemsetAsync(count, 0, sizeof(unsigned long long), st));
+    HIP_TRY(gps::before_epoch_launch(h, st));
+    if (struct_scan) {
+        const gps::TileShape s = gps::v4_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 8};
+        *resident = gps::v4_resident_workgroups(s, true);
+        HIP_TRY(gps::launch_csdl_u32x4(s, true, true, d_in, d_out, n, h->status, h->ticket, h->err, st, count, true));
+    } else {
+        gps::ScanParams p{};
+        const uint64_t tile = h->shape.tile();
+        p.in = d_in;
+        p.out = d_out;
+        p.n = n;
+        p.num_tiles = (uint32_t)((n + tile - 1) / tile);
+        p.vec_ok = aligned16(d_in) && aligned16(d_out);
+        p.status = h->status;
+        p.ticket = h->ticket;
+        p.err = h->err;
+        p.occupancy = count;
+        *resident = gps::pp_resident_workgroups(h->shape, true);
+        HIP_TRY(gps::launch_csdl_pp_u32(h->shape, true, p, st, true));
+    }
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, count, sizeof c, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *measured = (uint32_t)c;
+    return GPS_OK;
+}
+
 int gps_inclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream) {
     return scan_u32x4(h, d_in, d_out, n, true, stream);
 }

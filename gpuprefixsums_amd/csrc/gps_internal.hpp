@@ -13,6 +13,11 @@ namespace gps {
 // cache line. A handle holds one slot per kMinTileElems elements of max_n.
 constexpr uint64_t kSlotWords = 16;
 
+// Handle counter words: [0..3] the decoupled-fallback statistics, [4] the
+// occupancy estimate's count of workgroups that took a tile.
+constexpr uint64_t kCounterWords = 8;
+constexpr uint64_t kCounterOccupancy = 4;
+
 // Launch parameters shared by the single-pass uint32 kernels (CSDL, CSDLDF).
 struct ScanParams {
     const uint32_t* in;
@@ -27,6 +32,8 @@ struct ScanParams {
     uint32_t spin_limit;
     uint32_t stall_period;              // 0: no emulated stalls
     unsigned long long* counters;       // [polls, fallbacks, insertions, stall timeouts] or null
+    // persistent CSDL only: +1 per workgroup that takes a tile, or null
+    unsigned long long* occupancy;
 };
 
 struct TileShape {
@@ -41,7 +48,13 @@ bool shape_supported(TileShape s);
 // returns the HIP launch status.
 hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 // Persistent pipelined CSDL: grid = resident workgroups (<= num_tiles).
-hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
+// With grid_all_tiles, one workgroup is launched per tile instead: those that
+// start after the tickets run out exit at once (the occupancy estimate).
+hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream,
+                              bool grid_all_tiles = false);
+// Workgroups the persistent kernel keeps resident (the occupancy query times
+// the CU count); 0 for an unsupported shape or a failed query.
+uint32_t pp_resident_workgroups(TileShape s, bool inclusive);
 // Persistent CSDL with a control wave: block - 64 data threads per tile.
 uint32_t cw_tile_elems(TileShape s);
 hipError_t launch_csdl_cw_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
@@ -73,7 +86,9 @@ bool v4_shape_supported(TileShape s);
 // persistent: resident workgroups with the pipelined ticket loop; otherwise
 // one workgroup per tile.
 hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, bool persistent, const uint32_t* in, uint32_t* out,
-                             uint64_t n, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
+                             uint64_t n, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream,
+                             unsigned long long* occupancy = nullptr, bool grid_all_tiles = false);
+uint32_t v4_resident_workgroups(TileShape s, bool inclusive);
 
 // Clears `count` status words and arms the ticket word at epoch 1.
 hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream);

@@ -141,6 +141,7 @@ csdl_pp_u32(ScanParams p) {
     uint32_t tile = s_ticket[0];
     const uint32_t epoch = s_ticket[1];
     if (tile >= p.num_tiles) return;
+    if (threadIdx.x == 0) atomic_add_counter(p.occupancy, 1ull);
 
     u32x4 a[VEC], b[VEC];
     const uint64_t base = thread_base<BLOCK, VEC>(tile);
@@ -735,7 +736,7 @@ hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hip
 // Persistent grid: as many workgroups as fit on the device at once (the
 // occupancy query is made once per kernel instance and process).
 template <int B, int V, bool I>
-static hipError_t launch_pp(const ScanParams& p, hipStream_t stream) {
+static uint32_t pp_residency() {
     struct Residency {
         uint32_t workgroups = 0;
         Residency() {
@@ -748,20 +749,36 @@ static hipError_t launch_pp(const ScanParams& p, hipStream_t stream) {
         }
     };
     static const Residency r;
-    if (r.workgroups == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = p.num_tiles < r.workgroups ? p.num_tiles : r.workgroups;
+    return r.workgroups;
+}
+
+template <int B, int V, bool I>
+static hipError_t launch_pp(const ScanParams& p, hipStream_t stream, bool grid_all_tiles) {
+    const uint32_t resident = pp_residency<B, V, I>();
+    if (resident == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = (grid_all_tiles || p.num_tiles < resident) ? p.num_tiles : resident;
     hipLaunchKernelGGL((csdl_pp_u32<B, V, I>), dim3(grid), dim3(B), 0, stream, p);
     return hipGetLastError();
 }
 
-hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
-#define X(B, V)                                                                          \
-    if (s.block == B && s.vec == V) {                                                    \
-        return inclusive ? launch_pp<B, V, true>(p, stream) : launch_pp<B, V, false>(p, stream); \
+hipError_t launch_csdl_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream,
+                              bool grid_all_tiles) {
+#define X(B, V)                                                                                 \
+    if (s.block == B && s.vec == V) {                                                           \
+        return inclusive ? launch_pp<B, V, true>(p, stream, grid_all_tiles)                     \
+                         : launch_pp<B, V, false>(p, stream, grid_all_tiles);                   \
     }
     GPS_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+uint32_t pp_resident_workgroups(TileShape s, bool inclusive) {
+#define X(B, V) \
+    if (s.block == B && s.vec == V) return inclusive ? pp_residency<B, V, true>() : pp_residency<B, V, false>();
+    GPS_SHAPES(X)
+#undef X
+    return 0;
 }
 
 template <int DW, int V, bool I>

@@ -27,6 +27,7 @@ struct V4Params {
     uint64_t* status;  // 4 words per tile
     uint64_t* ticket;
     uint32_t* err;
+    unsigned long long* occupancy;  // persistent kernel: +1 per workgroup that takes a tile, or null
 };
 
 __device__ __forceinline__ void publish_v4(uint64_t* status, uint32_t tile, u32x4 v, uint64_t flag, uint32_t epoch) {
@@ -250,6 +251,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t tile = s_ticket[0];
     const uint32_t epoch = s_ticket[1];
     if (tile >= p.num_tiles) return;
+    if (threadIdx.x == 0) atomic_add_counter(p.occupancy, 1ull);
 
     u32x4 a[VEC], b[VEC];
     load_v4<BLOCK, VEC>(a, p, tile, wave, lane);
@@ -309,7 +311,7 @@ bool v4_shape_supported(TileShape s) {
 // Persistent grid: as many workgroups as fit on the device at once (the
 // occupancy query is made once per kernel instance and process).
 template <int B, int V, bool I>
-static hipError_t launch_pp_v4(const V4Params& p, hipStream_t stream) {
+static uint32_t v4_residency() {
     struct Residency {
         uint32_t workgroups = 0;
         Residency() {
@@ -322,14 +324,29 @@ static hipError_t launch_pp_v4(const V4Params& p, hipStream_t stream) {
         }
     };
     static const Residency r;
-    if (r.workgroups == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = p.num_tiles < r.workgroups ? p.num_tiles : r.workgroups;
+    return r.workgroups;
+}
+
+template <int B, int V, bool I>
+static hipError_t launch_pp_v4(const V4Params& p, hipStream_t stream, bool grid_all_tiles) {
+    const uint32_t resident = v4_residency<B, V, I>();
+    if (resident == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = (grid_all_tiles || p.num_tiles < resident) ? p.num_tiles : resident;
     hipLaunchKernelGGL((csdl_pp_u32x4<B, V, I>), dim3(grid), dim3(B), 0, stream, p);
     return hipGetLastError();
 }
 
+uint32_t v4_resident_workgroups(TileShape s, bool inclusive) {
+#define X(B, V) \
+    if (s.block == B && s.vec == V) return inclusive ? v4_residency<B, V, true>() : v4_residency<B, V, false>();
+    GPS_V4_SHAPES(X)
+#undef X
+    return 0;
+}
+
 hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, bool persistent, const uint32_t* in, uint32_t* out,
-                             uint64_t n, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream) {
+                             uint64_t n, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream,
+                             unsigned long long* occupancy, bool grid_all_tiles) {
     V4Params p{};
     p.in = in;
     p.out = out;
@@ -338,10 +355,12 @@ hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, bool persistent, const
     p.status = status;
     p.ticket = ticket;
     p.err = err;
+    p.occupancy = occupancy;
 #define X(B, V)                                                                                            \
     if (s.block == B && s.vec == V) {                                                                      \
         if (persistent)                                                                                    \
-            return inclusive ? launch_pp_v4<B, V, true>(p, stream) : launch_pp_v4<B, V, false>(p, stream); \
+            return inclusive ? launch_pp_v4<B, V, true>(p, stream, grid_all_tiles)                         \
+                             : launch_pp_v4<B, V, false>(p, stream, grid_all_tiles);                       \
         if (inclusive)                                                                                     \
             hipLaunchKernelGGL((csdl_u32x4<B, V, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);     \
         else                                                                                               \

@@ -129,6 +129,19 @@ int gps_graph_destroy(gps_graph* g);
 int gps_inclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 int gps_exclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 
+/* Occupancy estimate of the persistent scan, after the reference's
+ * occupancy test variants (GPUPrefixSumsWebGPUapis/SharedShaders/TestVariants/
+ * csdldf_occ.wgsl:70-82 and csdldf_struct_occ.wgsl, read back by
+ * Dawn/main.cpp:663-672): an inclusive scan of d_in into d_out (n uint32, or
+ * n structs when struct_scan != 0) launched with one workgroup per tile.
+ * Workgroups that find the tickets already gone exit at once, so *measured
+ * counts those that took a tile: the workgroups resident while the scan ran.
+ * *resident is the grid the normal persistent launch uses (the HIP occupancy
+ * query times the CU count). The scan's output is the ordinary result.
+ * Synchronises `stream`. Needs a GPS_ALGO_CSDL handle. */
+int gps_estimate_occupancy(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int struct_scan,
+                           void* stream, uint32_t* measured, uint32_t* resident);
+
 /* ---- floating-point inclusive scans (no reference counterpart) ----------- */
 
 /* FP32 or BF16 (raw uint16 bit patterns) input, FP32 output, accumulated in
