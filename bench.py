@@ -35,7 +35,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
 ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
-            "struct", "occupancy", "tuner"]
+            "struct", "occupancy", "survey", "tuner"]
 
 
 def log(msg: str) -> None:
@@ -312,6 +312,37 @@ class Legs:
             res["struct_n2p26"] = {"workgroups_that_took_a_tile": m, "resident_grid": r,
                                    "tiles": -(-(self.n // 4) // 8192)}
         out["occupancy"] = res
+
+    def survey(self, out):
+        """Survey networks (serial, wave / block Kogge-Stone, Sklansky,
+        Brent-Kung, raking, DPP, the register tile) scanning independent
+        segments of their own length over 2^26 of the input, inclusive: GB/s
+        at 8 B per element, each result checked against the oracle."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        n = 1 << 26
+        x = self.d_in[:n]
+        y = self.d_out[:n]
+        xh = x.cpu().numpy().view(np.uint32)
+        refs = {}
+        st = torch.cuda.current_stream()
+        res = {}
+        for name, alg in gps.SURVEY_NETWORKS.items():
+            seg = gps.survey_segment(alg)
+            for _ in range(2):
+                gps.survey_scan_u32(alg, x, y, n, True)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(10):
+                gps.survey_scan_u32(alg, x, y, n, True)
+            e1.record(st)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 10
+            if seg not in refs:
+                refs[seg] = oracle.segmented_scan_u32(xh, np.arange(0, n + 1, seg, dtype=np.uint64), True)
+            if not np.array_equal(y.cpu().numpy().view(np.uint32), refs[seg]):
+                raise RuntimeError(f"survey network {name} differs from the oracle")
+            res[name] = {"segment": seg, "gbps": round(gbps(n, ms), 1), "kernel_ms": round(ms, 4)}
+        out["survey_n2p26"] = res
 
     def tuner(self, out):
         """Learned tuner: regret of the MLP cost model's choices (inference on

@@ -263,6 +263,27 @@ def wave_primitive_u32(d_in, d_out, n: int, op: int, stream=None) -> None:
     check(load().gps_wave_primitive_u32(_ptr(d_in), _ptr(d_out), n, op, _stream(stream)))
 
 
+# ---- survey of wave and block scan networks, Survey.hlsl:49-855 -------------
+
+SURVEY_NETWORKS = {
+    "serial": 0, "wave_kogge_stone_lds": 1, "wave_kogge_stone_shuffle": 2, "wave_dpp": 3, "wave_raking": 4,
+    "block_kogge_stone": 5, "block_sklansky": 6, "block_brent_kung": 7, "block_wave_raking": 8,
+    "block_wave_sklansky": 9, "tile_registers": 10,
+}
+
+
+def survey_segment(alg: int) -> int:
+    """Segment length (elements) of a survey network."""
+    s = ctypes.c_uint32()
+    check(load().gps_survey_segment_u32(alg, ctypes.byref(s)))
+    return s.value
+
+
+def survey_scan_u32(alg: int, d_in, d_out, n: int, inclusive: bool = True, stream=None) -> None:
+    """Scans each segment of n elements independently with one survey network."""
+    check(load().gps_survey_scan_u32(alg, _ptr(d_in), _ptr(d_out), n, int(inclusive), _stream(stream)))
+
+
 # ---- vendor comparator (rocPRIM), CubDispatcher.cuh:32-76 ------------------
 
 def vendor_scan_temp_bytes(n: int, inclusive: bool = True) -> int:

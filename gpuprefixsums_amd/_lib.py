@@ -70,6 +70,8 @@ SIGNATURES = {
     "gps_segmented_scan_f32": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, _vp]),
     "gps_inclusive_scan_u32x4": (_int, [_vp, _vp, _vp, _u64, _vp]),
     "gps_exclusive_scan_u32x4": (_int, [_vp, _vp, _vp, _u64, _vp]),
+    "gps_survey_segment_u32": (_int, [_int, ctypes.POINTER(_u32)]),
+    "gps_survey_scan_u32": (_int, [_int, _vp, _vp, _u64, _int, _vp]),
     "gps_estimate_occupancy": (_int, [_vp, _vp, _vp, _u64, _int, _vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "gps_segmented_scan_bf16_f32": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, _vp]),
     "gps_mlp_param_count": (_u64, [_u32, _u32]),

@@ -33,6 +33,7 @@ LIB_SOURCES = [
     "mlp.hip",
     "vendor_rocprim.hip",
     "scan_u32x4.hip",
+    "survey.hip",
     "capi.cpp",
     "capi_ext.cpp",
     "dispatchers.cpp",

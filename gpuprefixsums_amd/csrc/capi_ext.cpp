@@ -204,6 +204,27 @@ int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_ou
     return segscan_fp(h, d_in, d_out, d_offsets, num_segments, n, true, compensated, stream);
 }
 
+int gps_survey_segment_u32(int alg, uint32_t* segment) {
+    if (!segment) return fail(GPS_ERR_INVALID, "null argument");
+    const uint32_t s = gps::survey_segment(alg);
+    if (!s) return fail(GPS_ERR_INVALID, "unknown survey network %d", alg);
+    *segment = s;
+    return GPS_OK;
+}
+
+int gps_survey_scan_u32(int alg, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream) {
+    const uint32_t seg = gps::survey_segment(alg);
+    if (!seg) return fail(GPS_ERR_INVALID, "unknown survey network %d", alg);
+    if (n % seg) return fail(GPS_ERR_INVALID, "n=%llu is not a multiple of the segment (%u)", (unsigned long long)n, seg);
+    if (n == 0) return GPS_OK;
+    if (!d_in || !d_out) return fail(GPS_ERR_INVALID, "null data pointer");
+    const uintptr_t mask = alg == gps::kSurveyTileRegisters ? 15u : 3u;
+    if ((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) & mask)
+        return fail(GPS_ERR_INVALID, "buffers must be %u-byte aligned", (unsigned)mask + 1u);
+    HIP_TRY(gps::launch_survey_u32(alg, inclusive != 0, d_in, d_out, n, static_cast<hipStream_t>(stream)));
+    return GPS_OK;
+}
+
 int gps_estimate_occupancy(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int struct_scan,
                            void* stream, uint32_t* measured, uint32_t* resident) {
     if (!h || !measured || !resident) return fail(GPS_ERR_INVALID, "null argument");

@@ -90,6 +90,27 @@ hipError_t launch_csdl_u32x4(TileShape s, bool inclusive, bool persistent, const
                              unsigned long long* occupancy = nullptr, bool grid_all_tiles = false);
 uint32_t v4_resident_workgroups(TileShape s, bool inclusive);
 
+// Survey of wave and block scan networks over fixed-length segments
+// (values match gps_survey_alg in gps.h).
+enum SurveyAlg : int {
+    kSurveySerial = 0,
+    kSurveyWaveKoggeStoneLds = 1,
+    kSurveyWaveKoggeStoneShuffle = 2,
+    kSurveyWaveDpp = 3,
+    kSurveyWaveRaking = 4,
+    kSurveyBlockKoggeStone = 5,
+    kSurveyBlockSklansky = 6,
+    kSurveyBlockBrentKung = 7,
+    kSurveyBlockWaveRaking = 8,
+    kSurveyBlockWaveSklansky = 9,
+    kSurveyTileRegisters = 10,
+    kSurveyCount = 11,
+};
+// Segment length in elements (0 for an unknown network).
+uint32_t survey_segment(int alg);
+hipError_t launch_survey_u32(int alg, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
+                             hipStream_t stream);
+
 // Clears `count` status words and arms the ticket word at epoch 1.
 hipError_t launch_reset_status(uint64_t* status, uint64_t count, uint64_t* ticket, hipStream_t stream);
 

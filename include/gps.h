@@ -142,6 +142,31 @@ int gps_exclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_ou
 int gps_estimate_occupancy(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int struct_scan,
                            void* stream, uint32_t* measured, uint32_t* resident);
 
+/* ---- survey of wave and block scan networks -------------------------------
+ * The reference's survey (GPUPrefixSumsD3D12/Shaders/Survey.hlsl:49-855):
+ * serial, wave and block Kogge-Stone, Sklansky, Brent-Kung-Blelloch and
+ * raking scans, here as wave64 device functions run by a batch kernel over
+ * independent segments of a fixed length per network. */
+typedef enum gps_survey_alg {
+    GPS_SURVEY_SERIAL = 0,                  /* one thread per 64-element segment */
+    GPS_SURVEY_WAVE_KOGGE_STONE_LDS = 1,    /* 64: through LDS, no cross-lane ops */
+    GPS_SURVEY_WAVE_KOGGE_STONE_SHUFFLE = 2,/* 64: ds_bpermute shuffles */
+    GPS_SURVEY_WAVE_DPP = 3,                /* 64: DPP row_shr / row_bcast (the scans' own) */
+    GPS_SURVEY_WAVE_RAKING = 4,             /* 1024: one wave, 16 DPP rounds with a carry */
+    GPS_SURVEY_BLOCK_KOGGE_STONE = 5,       /* 256: LDS, one element per thread */
+    GPS_SURVEY_BLOCK_SKLANSKY = 6,          /* 256 */
+    GPS_SURVEY_BLOCK_BRENT_KUNG = 7,        /* 256: Blelloch upsweep / downsweep */
+    GPS_SURVEY_BLOCK_WAVE_RAKING = 8,       /* 256: DPP wave scans + wave totals */
+    GPS_SURVEY_BLOCK_WAVE_SKLANSKY = 9,     /* 256: DPP wave scans + Sklansky fans */
+    GPS_SURVEY_TILE_REGISTERS = 10          /* 4096: the scans' register tile (256 x 4 uint4) */
+} gps_survey_alg;
+/* Segment length (elements) of a network. */
+int gps_survey_segment_u32(int alg, uint32_t* segment);
+/* Scans every segment of d_in into d_out independently (inclusive or
+ * exclusive, mod 2^32); n must be a multiple of the segment length. 4-byte
+ * aligned buffers (16-byte for GPS_SURVEY_TILE_REGISTERS). One launch. */
+int gps_survey_scan_u32(int alg, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream);
+
 /* ---- floating-point inclusive scans (no reference counterpart) ----------- */
 
 /* FP32 or BF16 (raw uint16 bit patterns) input, FP32 output, accumulated in

@@ -63,3 +63,14 @@ def test_binding_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "missing.so")
     with pytest.raises(ImportError):
         _lib.load()
+
+
+def test_survey_segment_lengths():
+    # Host-only query: no device work.
+    import gpuprefixsums_amd as gps
+    want = {"serial": 64, "wave_kogge_stone_lds": 64, "wave_kogge_stone_shuffle": 64, "wave_dpp": 64,
+            "wave_raking": 1024, "block_kogge_stone": 256, "block_sklansky": 256, "block_brent_kung": 256,
+            "block_wave_raking": 256, "block_wave_sklansky": 256, "tile_registers": 4096}
+    assert {k: gps.survey_segment(v) for k, v in gps.SURVEY_NETWORKS.items()} == want
+    with pytest.raises(gps.GpsError):
+        gps.survey_segment(len(want))
