@@ -226,10 +226,11 @@ class Legs:
 
     def fp(self, out):
         """n = 2^27 inclusive FP scans: FP32 ~ N(0,1) (seed 4) and BF16 with
-        magnitudes 10^U(-4,4) and random sign (seed 5), FP32 accumulation with
-        and without two-sum compensated carries; GB/s (8 B/elem FP32, 6 B/elem
-        BF16 in + FP32 out) and max / median relative error against the FP64
-        oracle, rel_i = |y_i - y64_i| / max(|y64_i|, 1)."""
+        magnitudes 10^U(-4,4) and random sign (seed 5); carries as two-float
+        TwoSum pairs ("compensated"), as FP64 ("f64carry") or as plain FP32
+        ("plain"); GB/s (8 B/elem FP32, 6 B/elem BF16 in + FP32 out) and max /
+        median relative error against the FP64 oracle,
+        rel_i = |y_i - y64_i| / max(|y64_i|, 1)."""
         gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
         n = 1 << 27
         st = torch.cuda.current_stream()
@@ -248,7 +249,7 @@ class Legs:
                     din = torch.from_numpy(x.view(np.int16)).cuda()
                     fn, bpe = h.inclusive_bf16_f32, 6
                 den = np.maximum(np.abs(y64), 1.0)
-                for comp in (True, False):
+                for comp in (gps.FP_PAIR, gps.FP_F64, gps.FP_PLAIN):
                     for _ in range(3):
                         fn(din, dout, n, comp)
                     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -259,7 +260,8 @@ class Legs:
                     torch.cuda.synchronize()
                     ms = e0.elapsed_time(e1) / 10
                     rel = np.abs(dout.cpu().numpy().astype(np.float64) - y64) / den
-                    res[f"{kind}_{'compensated' if comp else 'plain'}"] = {
+                    mode = {gps.FP_PAIR: "compensated", gps.FP_F64: "f64carry", gps.FP_PLAIN: "plain"}[comp]
+                    res[f"{kind}_{mode}"] = {
                         "gbps": round(n * bpe / ms / 1e6, 1), "max_rel_err": float(rel.max()),
                         "median_rel_err": float(np.median(rel))}
                 del x, y64, din, den

@@ -110,7 +110,8 @@ class ScanHandle:
         check(load().gps_exclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
 
     def scan(self, d_in, d_out, n: Optional[int] = None, inclusive: bool = True, stream=None) -> None:
-        (self.inclusive if inclusive else self.exclusive)(d_in, d_out, n, stream)
+        n = _numel(d_in) if n is None else n
+        check(load().gps_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive), _stream(stream)))
 
     def time(self, d_in, d_out, n: int, inclusive: bool = True, iters: int = 10, stream=None) -> list:
         ms = (ctypes.c_float * iters)()
@@ -132,15 +133,16 @@ class ScanHandle:
                                             _stream(stream), ctypes.byref(measured), ctypes.byref(resident)))
         return measured.value, resident.value
 
-    def inclusive_f32(self, d_in, d_out, n: Optional[int] = None, compensated: bool = True, stream=None) -> None:
-        """FP32 -> FP32 inclusive scan (two-sum compensated carries unless compensated=False)."""
+    def inclusive_f32(self, d_in, d_out, n: Optional[int] = None, compensated=True, stream=None) -> None:
+        """FP32 -> FP32 inclusive scan. compensated: False/0 plain FP32 carries,
+        True/1 two-float TwoSum pairs, FP_F64 (2) FP64 carries."""
         n = _numel(d_in) if n is None else n
         check(load().gps_inclusive_scan_f32(self.handle, _ptr(d_in), _ptr(d_out), n, int(compensated),
                                             _stream(stream)))
 
-    def inclusive_bf16_f32(self, d_in, d_out, n: Optional[int] = None, compensated: bool = True,
+    def inclusive_bf16_f32(self, d_in, d_out, n: Optional[int] = None, compensated=True,
                            stream=None) -> None:
-        """BF16 (uint16 bits) -> FP32 inclusive scan."""
+        """BF16 (uint16 bits) -> FP32 inclusive scan (compensated as inclusive_f32)."""
         n = _numel(d_in) if n is None else n
         check(load().gps_inclusive_scan_bf16_f32(self.handle, _ptr(d_in), _ptr(d_out), n, int(compensated),
                                                  _stream(stream)))
@@ -255,6 +257,9 @@ def validate_ones_u32(d_scan, n: int, inclusive: bool = True, stream=None) -> in
 
 
 WAVE_INCLUSIVE_SCAN, WAVE_SUM, WAVE_EXCLUSIVE_SCAN = 0, 1, 2
+
+# FP scan carries (gps.h GPS_FP_*).
+FP_PLAIN, FP_PAIR, FP_F64 = 0, 1, 2
 
 
 def wave_primitive_u32(d_in, d_out, n: int, op: int, stream=None) -> None:

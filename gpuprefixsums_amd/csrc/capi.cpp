@@ -226,6 +226,10 @@ int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out,
     return scan_u32(h, d_in, d_out, n, false, stream);
 }
 
+int gps_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream) {
+    return scan_u32(h, d_in, d_out, n, inclusive != 0, stream);
+}
+
 int gps_get_counters(gps_handle* h, uint64_t out[4]) {
     if (!h || !out) return fail(GPS_ERR_INVALID, "null argument");
     unsigned long long c[4];

@@ -74,7 +74,9 @@ hipError_t launch_segscan_u32(TileShape s, bool inclusive, bool persistent, cons
 
 // FP32 / BF16-input inclusive scans with FP32 output (2 status words per tile).
 bool fp_shape_supported(TileShape s);
-hipError_t launch_scan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out, uint64_t n,
+// Carries of the FP scans (values match GPS_FP_* in gps.h).
+enum FpCarry : int { kFpPlain = 0, kFpPair = 1, kFpF64 = 2 };
+hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, const void* in, float* out, uint64_t n,
                           uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
 // Batched segmented FP inclusive scan (one launch; same shapes as launch_scan_fp).
 hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out,

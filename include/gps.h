@@ -90,6 +90,9 @@ int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems);
  * DispatchKernelsExclusive (...Dispatcher.cuh:177-185). */
 int gps_inclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
+/* Both in one entry point: inclusive != 0 selects the inclusive scan (the
+ * form an FFI binds when it passes the direction as a flag). */
+int gps_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream);
 
 /* ---- batched segmented scan (no reference counterpart) ------------------- */
 
@@ -169,11 +172,19 @@ int gps_survey_scan_u32(int alg, const uint32_t* d_in, uint32_t* d_out, uint64_t
 
 /* ---- floating-point inclusive scans (no reference counterpart) ----------- */
 
-/* FP32 or BF16 (raw uint16 bit patterns) input, FP32 output, accumulated in
- * FP32. compensated != 0 carries every partial sum above the element level
- * (lane, wave, tile and the inter-tile lookback chain) as a two-float pair
- * combined with TwoSum, and rounds each output once; compensated == 0 is the
- * plain FP32 chained scan. One kernel launch; needs a GPS_ALGO_CSDL handle. */
+/* FP32 or BF16 (raw uint16 bit patterns) input, FP32 output. `compensated`
+ * selects how every partial sum above the element level (lane, wave, tile and
+ * the inter-tile lookback chain) is carried:
+ *   GPS_FP_PLAIN (0): one FP32 float, the plain chained scan;
+ *   GPS_FP_PAIR  (1): a two-float pair combined with TwoSum, each output
+ *                     rounded once;
+ *   GPS_FP_F64   (2): one FP64 value, each output rounded once (same
+ *                     accuracy as the pair, fewer instructions).
+ * Other values return GPS_ERR_INVALID. One kernel launch; needs a
+ * GPS_ALGO_CSDL handle. */
+#define GPS_FP_PLAIN 0
+#define GPS_FP_PAIR 1
+#define GPS_FP_F64 2
 int gps_inclusive_scan_f32(gps_handle* h, const float* d_in, float* d_out, uint64_t n, int compensated,
                            void* stream);
 int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, uint64_t n, int compensated,
@@ -207,9 +218,10 @@ int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch
 /* Batched segmented FP inclusive scan in one launch over the concatenated
  * segments: segment s covers [d_offsets[s], d_offsets[s+1]) (device array of
  * num_segments + 1 entries, d_offsets[0] = 0, d_offsets[S] = n; empty
- * segments allowed). Same input/output types and compensation as the plain
- * FP scans; the carry restarts at every segment head at every level (lane,
- * wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. */
+ * segments allowed). Same input/output types as the plain FP scans;
+ * compensated is GPS_FP_PLAIN or GPS_FP_PAIR (GPS_FP_F64 returns
+ * GPS_ERR_UNSUPPORTED). The carry restarts at every segment head at every
+ * level (lane, wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. */
 int gps_segmented_scan_f32(gps_handle* h, const float* d_in, float* d_out, const uint64_t* d_offsets,
                            uint64_t num_segments, uint64_t n, int compensated, void* stream);
 int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, const uint64_t* d_offsets,

@@ -5,8 +5,10 @@ Error measures (stated with every result):
           inputs have unit median magnitude, so prefixes near a zero crossing
           are judged in absolute terms)
   S_i   = sum_{j <= i} |x_j|                 (for normwise bounds)
+Modes: plain FP32 carries (0), two-float TwoSum pairs (1), FP64 carries (2).
 Bounds asserted:
-  compensated:   |y_i - y64_i| <= 2^-23 |y64_i| + 2^-36 S_i
+  compensated (pairs or FP64):
+                 |y_i - y64_i| <= 2^-23 |y64_i| + 2^-36 S_i
                  (one FP32 rounding of the exact prefix, plus the u^2-level
                  error of the pair arithmetic)
   uncompensated: |y_i - y64_i| <= (64 + 2 (t_i + 1)) 2^-24 S_i
@@ -23,6 +25,9 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 import gpuprefixsums_amd as gps  # noqa: E402
+
+MODES = [gps.FP_PAIR, gps.FP_PLAIN, gps.FP_F64]
+MODE_IDS = ["pair", "plain", "f64"]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -54,7 +59,7 @@ def check_bounds(y, y64, S, comp, tile):
     return float(np.max(err / np.maximum(np.abs(y64), 1.0))), float(np.median(err / np.maximum(np.abs(y64), 1.0)))
 
 
-@pytest.mark.parametrize("comp", [True, False])
+@pytest.mark.parametrize("comp", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("bf16", [False, True])
 def test_integer_valued_inputs_are_exact(bf16, comp):
     # |prefix| <= 8 * 2^20 = 2^23: every partial sum is exact in FP32.
@@ -68,7 +73,7 @@ def test_integer_valued_inputs_are_exact(bf16, comp):
 
 
 @pytest.mark.parametrize("shape", [(256, 4), (1024, 4), (1024, 8), (0, 0)])
-@pytest.mark.parametrize("comp", [True, False])
+@pytest.mark.parametrize("comp", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("n", [1, 3, 4095, 4097, 65537, (1 << 22) + 7])
 def test_f32_normal_bounds(n, comp, shape):
     x = oracle.gen_f32_normal(4, n)
@@ -81,7 +86,7 @@ def test_f32_normal_bounds(n, comp, shape):
     check_bounds(y, y64, S, comp, tile)
 
 
-@pytest.mark.parametrize("comp", [True, False])
+@pytest.mark.parametrize("comp", MODES, ids=MODE_IDS)
 def test_bf16_logmag_bounds(comp):
     n = (1 << 22) + 3
     b = oracle.gen_bf16_logmag(5, n)
@@ -98,12 +103,15 @@ def test_compensation_beats_plain_fp32():
     x = oracle.gen_f32_normal(4, n)
     y64 = oracle.scan_f32_f64(x)
     with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
-        yc = run(h, x, False, True)
-        yp = run(h, x, False, False)
+        yc = run(h, x, False, gps.FP_PAIR)
+        yd = run(h, x, False, gps.FP_F64)
+        yp = run(h, x, False, gps.FP_PLAIN)
     ec = np.abs(yc - y64) / np.maximum(np.abs(y64), 1.0)
+    ed = np.abs(yd - y64) / np.maximum(np.abs(y64), 1.0)
     ep = np.abs(yp - y64) / np.maximum(np.abs(y64), 1.0)
     assert np.median(ec) < np.median(ep) and ec.max() < ep.max()
-    assert ec.max() <= 2.0 ** -23
+    assert np.median(ed) < np.median(ep) and ed.max() < ep.max()
+    assert ec.max() <= 2.0 ** -23 and ed.max() <= 2.0 ** -23
 
 
 @pytest.mark.parametrize("bf16", [False, True])
@@ -118,11 +126,25 @@ def test_config4_n2p27(bf16):
         y64, S = oracle.scan_f32_f64(x), oracle.abs_scan_f32_f64(x)
     report = {}
     with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
-        for comp in (True, False):
+        for comp in MODES:
             y = run(h, x, bf16, comp)
             report[comp] = check_bounds(y, y64, S, comp, h.tile)
             del y
         assert h.device_error() == 0
-    print(f"n=2^27 {'bf16' if bf16 else 'f32'}: compensated max/median rel {report[True]}, "
-          f"plain {report[False]}")
-    assert report[True][0] <= 2.0 ** -23 and report[True][0] < report[False][0]
+    print(f"n=2^27 {'bf16' if bf16 else 'f32'}: pair max/median rel {report[gps.FP_PAIR]}, "
+          f"FP64 carries {report[gps.FP_F64]}, plain {report[gps.FP_PLAIN]}")
+    for comp in (gps.FP_PAIR, gps.FP_F64):
+        assert report[comp][0] <= 2.0 ** -23 and report[comp][0] < report[gps.FP_PLAIN][0]
+
+
+def test_carry_mode_errors():
+    d = torch.zeros(1024, dtype=torch.float32, device="cuda")
+    off = torch.tensor([0, 1024], dtype=torch.int64, device="cuda")
+    with gps.ScanHandle(gps.ALGO_CSDL, 1024) as h:
+        with pytest.raises(gps.GpsError, match="compensated"):
+            h.inclusive_f32(d, d, 1024, 3)
+        with pytest.raises(gps.GpsError, match="FP64"):
+            h.segmented_fp(d, d, off, 1, 1024, compensated=gps.FP_F64)
+        # the per-segment loop runs the unsegmented kernel, FP64 carries included
+        h.fp_per_segment(d, d, [0, 1024], compensated=gps.FP_F64)
+        torch.cuda.synchronize()
