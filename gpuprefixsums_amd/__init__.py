@@ -25,7 +25,9 @@ __all__ = [
     "ChainedScanDecoupledLookbackDispatcher", "ReduceThenScanDispatcher",
     "EmulatedDeadlockingDispatcher", "fill_u32", "fill_random_u32", "validate_scan_u32",
     "validate_ones_u32", "load", "vendor_scan_temp_bytes", "vendor_scan_u32", "vendor_time_scan_u32",
-    "CubDispatcher",
+    "CubDispatcher", "SCHEDULE_DEFAULT", "SCHEDULE_TILE", "SCHEDULE_PERSISTENT", "SCHEDULE_CONTROL",
+    "FP_PLAIN", "FP_PAIR", "FP_F64", "WAVE_INCLUSIVE_SCAN", "WAVE_SUM", "WAVE_EXCLUSIVE_SCAN",
+    "wave_primitive_u32", "SURVEY_NETWORKS", "survey_segment", "survey_scan_u32",
 ]
 
 
