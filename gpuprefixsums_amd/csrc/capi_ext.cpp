@@ -18,9 +18,9 @@ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 bool seg_persistent(const gps_handle* h) { return h->schedule != GPS_SCHEDULE_TILE; }
 
 // Segmented kernels use the handle's tile shape when it is one of theirs,
-// otherwise 1024 threads x 4 uint4 (16384-element tiles).
+// otherwise 1024 threads x 8 uint4 (32768-element tiles).
 gps::TileShape seg_shape(const gps_handle* h) {
-    return gps::seg_shape_supported(h->shape, seg_persistent(h)) ? h->shape : gps::TileShape{1024, 4};
+    return gps::seg_shape_supported(h->shape, seg_persistent(h)) ? h->shape : gps::TileShape{1024, 8};
 }
 
 int check_seg(gps_handle* h, const void* in, const void* out, const void* offsets, uint64_t n) {

@@ -37,12 +37,6 @@ struct SegParams {
     uint32_t* err;
 };
 
-// Segmented combine of (has_head, value) pairs, left then right.
-__device__ __forceinline__ void seg_combine(uint32_t& f, uint32_t& v, uint32_t f2, uint32_t v2) {
-    v = f2 ? v2 : v + v2;
-    f |= f2;
-}
-
 // Segmented scan of one wave's run (256 * VEC elements starting at tile
 // index run_base) in registers, with carry-in 0, under the tile's head
 // bitmap. On return d holds each element's segmented inclusive (INCL) or
@@ -170,13 +164,8 @@ __global__ __launch_bounds__(BLOCK) void segscan_u32(SegParams p) {
 
     // Tile level: segmented prefix of the waves before this one, and the
     // tile's own (flag, value).
-    uint32_t pre_f = 0, pre_v = 0, tile_f = 0, tile_v = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < C::kWaves; ++q) {
-        const uint32_t fq = s_wave_flag[q], vq = s_wave_val[q];
-        if (q < wave) seg_combine(pre_f, pre_v, fq, vq);
-        seg_combine(tile_f, tile_v, fq, vq);
-    }
+    uint32_t pre_f, pre_v, tile_f, tile_v;
+    combine_waves_seg<C::kWaves>(s_wave_flag, s_wave_val, wave, pre_f, pre_v, tile_f, tile_v);
     const bool starts_with_head = (s_heads[0] & 1u) != 0u;
     if (wave == 0) {
         if (tile_f) {
@@ -267,13 +256,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             s_wave_flag[wave] = carry_flag;
         }
         __syncthreads();
-        uint32_t pre_f = 0, pre_v = 0, tile_f = 0, tile_v = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < C::kWaves; ++q) {
-            const uint32_t fq = s_wave_flag[q], vq = s_wave_val[q];
-            if (q < wave) seg_combine(pre_f, pre_v, fq, vq);
-            seg_combine(tile_f, tile_v, fq, vq);
-        }
+        uint32_t pre_f, pre_v, tile_f, tile_v;
+        combine_waves_seg<C::kWaves>(s_wave_flag, s_wave_val, wave, pre_f, pre_v, tile_f, tile_v);
         const bool need_lookback = tile != 0 && (s_heads[cur][0] & 1u) == 0u;
         if (threadIdx.x == 0) {
             // a tile with a head knows its carry-out: publish it inclusive now
@@ -329,9 +313,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
-// Shapes of the one-workgroup-per-tile kernel, and of the persistent one
-// (which holds two tiles per thread: at most 4 uint4 each, so it stays in
-// 128 VGPRs at 4 waves per SIMD without spilling).
+// Shapes of the one-workgroup-per-tile kernel, and of the persistent one,
+// which holds two tiles per thread within 128 VGPRs at 4 waves per SIMD. At
+// 1024 x 8 the exclusive form spills 2 VGPRs and still measured 11% faster at
+// C3 than 1024 x 4 (0.080 against 0.090 ms).
 #define GPS_SEG_SHAPES(X) \
     X(256, 4)             \
     X(512, 4)             \
@@ -342,7 +327,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     X(256, 4)                \
     X(512, 4)                \
     X(1024, 2)               \
-    X(1024, 4)
+    X(1024, 4)               \
+    X(1024, 8)
 
 bool seg_shape_supported(TileShape s, bool persistent) {
 #define X(B, V) \

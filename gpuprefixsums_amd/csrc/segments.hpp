@@ -78,4 +78,46 @@ __device__ __forceinline__ uint32_t head_nibble(const uint32_t* s_heads, uint32_
     return (s_heads[e >> 5] >> (e & 31u)) & 0xFu;
 }
 
+// Inclusive segmented wave scan of (f, v) under
+//     (f1, v1) + (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2),
+// on the plain scan's DPP schedule. Lanes a step does not feed receive
+// (0, 0), the identity.
+__device__ __forceinline__ void wave_seg_scan_u32(uint32_t& f, uint32_t& v) {
+#define GPS_SEG_U32_STEP(CTRL, RM, BC)                                                            \
+    {                                                                                             \
+        const uint32_t yv = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xf, BC);  \
+        const uint32_t yf = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)f, CTRL, RM, 0xf, BC);  \
+        v = f ? v : yv + v;                                                                       \
+        f |= yf;                                                                                  \
+    }
+    GPS_SEG_U32_STEP(0x111, 0xf, true)   // row_shr:1
+    GPS_SEG_U32_STEP(0x112, 0xf, true)   // row_shr:2
+    GPS_SEG_U32_STEP(0x114, 0xf, true)   // row_shr:4
+    GPS_SEG_U32_STEP(0x118, 0xf, true)   // row_shr:8
+    GPS_SEG_U32_STEP(0x142, 0xa, false)  // row_bcast:15
+    GPS_SEG_U32_STEP(0x143, 0xc, false)  // row_bcast:31
+#undef GPS_SEG_U32_STEP
+}
+
+// Segmented combine of the waves' (flag, total) pairs from LDS by one DPP
+// pass (lane q holds wave q's pair): this wave's exclusive prefix and the
+// tile's pair. A serial loop over the totals would hold 2 * WAVES more
+// registers.
+template <int WAVES>
+__device__ __forceinline__ void combine_waves_seg(const uint32_t* s_flag, const uint32_t* s_val, uint32_t wave,
+                                                  uint32_t& pre_f, uint32_t& pre_v, uint32_t& tile_f,
+                                                  uint32_t& tile_v) {
+    const uint32_t lane = lane_id();
+    const uint32_t q = lane < (uint32_t)WAVES ? lane : 0u;
+    uint32_t f = lane < (uint32_t)WAVES ? s_flag[q] : 0u;
+    uint32_t v = lane < (uint32_t)WAVES ? s_val[q] : 0u;
+    wave_seg_scan_u32(f, v);
+    const uint32_t ef = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)f, 0x138, 0xf, 0xf, true);  // wave_shr:1
+    const uint32_t ev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, true);
+    pre_f = readlane(ef, (int)wave);
+    pre_v = readlane(ev, (int)wave);
+    tile_f = readlane(f, WAVES - 1);
+    tile_v = readlane(v, WAVES - 1);
+}
+
 }  // namespace gps

@@ -48,7 +48,7 @@ class Config:
 
 
 U32_CONFIGS = (
-    [Config("batched", b, v) for b, v in ((256, 4), (512, 4), (1024, 2), (1024, 4))]
+    [Config("batched", b, v) for b, v in ((256, 4), (512, 4), (1024, 2), (1024, 4), (1024, 8))]
     + [Config("loop_csdl", b, v) for b, v in ((1024, 8), (1024, 4), (512, 8), (256, 4))]
     + [Config("loop_rts", b, v) for b, v in ((512, 2), (256, 4), (1024, 2))]
     + [Config("loop_csdldf", 1024, 4, s) for s in (1, 4, 32)]
@@ -67,7 +67,7 @@ def default_config(dtype: str, segments: int) -> Config:
     floating point the default FP shape (one launch, plain or segmented)."""
     if dtype != "u32":
         return Config("loop_fp", 1024, 8) if segments == 1 else Config("batched_fp", 1024, 8)
-    return Config("loop_csdl", 1024, 8) if segments == 1 else Config("batched", 1024, 4)
+    return Config("loop_csdl", 1024, 8) if segments == 1 else Config("batched", 1024, 8)
 
 
 @dataclass

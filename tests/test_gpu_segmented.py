@@ -44,8 +44,8 @@ SCHEDULES = {"persistent": gps.SCHEDULE_PERSISTENT, "tile": gps.SCHEDULE_TILE}
 @pytest.mark.parametrize("shape", [(0, 0), (256, 4), (512, 4), (1024, 2), (1024, 4), (1024, 8)])
 @pytest.mark.parametrize("name,off", LAYOUTS, ids=[x[0] for x in LAYOUTS])
 def test_segmented_matches_oracle(name, off, shape, schedule, inclusive):
-    """Both segmented kernels: the persistent one (default schedule; a 1024x8
-    handle runs it at 1024x4) and one workgroup per tile."""
+    """Both segmented kernels: the persistent one (default schedule) and one
+    workgroup per tile, at every shape (the default handle's is 1024x8)."""
     n = int(off[-1])
     x = oracle.gen_u32(17 + n, n, 32)
     want = oracle.segmented_scan_u32(x, off, inclusive)
