@@ -160,7 +160,7 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     gps_graph* g = new (std::nothrow) gps_graph();
     if (!g) {
-        hipStreamDestroy(s);
+        (void)hipStreamDestroy(s);
         return fail(GPS_ERR_INVALID, "out of host memory");
     }
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -179,10 +179,10 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
         if (rc != GPS_OK && e == hipSuccess) e = hipErrorLaunchFailure;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
-    hipStreamDestroy(s);
+    (void)hipStreamDestroy(s);
     if (e != hipSuccess) {
-        if (g->exec) hipGraphExecDestroy(g->exec);
-        if (g->graph) hipGraphDestroy(g->graph);
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
         delete g;
         return hip_fail(e, "segmented graph capture");
     }
@@ -305,8 +305,8 @@ int gps_graph_launch(gps_graph* g, void* stream) {
 
 int gps_graph_destroy(gps_graph* g) {
     if (!g) return fail(GPS_ERR_INVALID, "null graph");
-    if (g->exec) hipGraphExecDestroy(g->exec);
-    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
     return GPS_OK;
 }

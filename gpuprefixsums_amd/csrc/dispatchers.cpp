@@ -30,15 +30,15 @@ DispatcherBase::DispatcherBase(int algo, uint32_t maxSize, gps_tuning tuning)
     hipError_t e = hipMalloc(&m_scan, (size_t)std::max(k_maxSize, 4u) * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&m_scratch, (size_t)std::max(k_maxSize, 4u) * sizeof(uint32_t));
     if (e != hipSuccess) {
-        hipFree(m_scan);
+        (void)hipFree(m_scan);
         gps_destroy(m_handle);
         check_hip(e, "hipMalloc");
     }
 }
 
 DispatcherBase::~DispatcherBase() {
-    hipFree(m_scan);
-    hipFree(m_scratch);
+    (void)hipFree(m_scan);
+    (void)hipFree(m_scratch);
     gps_destroy(m_handle);
 }
 
@@ -133,8 +133,8 @@ double DispatcherBase::BatchTiming(bool inclusive, uint32_t size, uint32_t batch
         if (i) total_ms += ms;
         if ((i & 15) == 0) printf(". ");
     }
-    hipEventDestroy(start);
-    hipEventDestroy(stop);
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
     const double secs = total_ms / 1000.0;
     const double keys = secs > 0 ? (double)size * batchCount / secs : 0.0;
     printf("\nTotal time elapsed: %f\n", secs);
@@ -179,7 +179,7 @@ CubDispatcher::CubDispatcher(uint32_t size) : k_maxSize(size) {
     check_hip(hipMalloc(&m_scan, (size_t)std::max(k_maxSize, 1u) * sizeof(uint32_t)), "hipMalloc");
 }
 
-CubDispatcher::~CubDispatcher() { hipFree(m_scan); }
+CubDispatcher::~CubDispatcher() { (void)hipFree(m_scan); }
 
 double CubDispatcher::BatchTimingCubChainedScan(uint32_t size, uint32_t batchCount) {
     if (size > k_maxSize || size == 0) {
@@ -214,9 +214,9 @@ double CubDispatcher::BatchTimingCubChainedScan(uint32_t size, uint32_t batchCou
     }
     uint64_t errors = 0;
     if (rc == GPS_OK && e == hipSuccess) rc = gps_validate_ones_u32(m_scan, size, 1, nullptr, &errors);
-    if (start) hipEventDestroy(start);
-    if (stop) hipEventDestroy(stop);
-    hipFree(temp);
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+    (void)hipFree(temp);
     check_hip(e, "rocPRIM batch timing");
     check(rc, "rocPRIM batch timing");
     if (errors) throw Error(GPS_ERR_DEVICE, "rocPRIM scan of all-ones input has mismatches");

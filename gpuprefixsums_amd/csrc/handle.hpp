@@ -20,6 +20,7 @@ struct gps_handle {
     uint32_t spin_limit = 4;
     uint32_t stall_period = 0;
     uint32_t schedule = GPS_SCHEDULE_TILE;
+    bool size_adaptive = false;  // CSDL with default shape and schedule: both picked per call from n
     uint32_t max_tiles = 0;     // status slots allocated (one per kMinTileElems elements)
     int device = 0;
     uint64_t* status = nullptr;

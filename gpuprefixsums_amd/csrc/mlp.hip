@@ -143,7 +143,7 @@ void free_mlp(gps_mlp* m) {
     if (!m) return;
     float* ptrs[] = {m->params, m->grads, m->m, m->v, m->x, m->t, m->z1, m->a1, m->z2, m->a2,
                      m->y, m->dy, m->dz2, m->dz1, m->partial, m->loss};
-    for (float* p : ptrs) hipFree(p);
+    for (float* p : ptrs) (void)hipFree(p);
     delete m;
 }
 

@@ -287,8 +287,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         store_v4<BLOCK, VEC>(p, a, (u32x4){s_prefix[0], s_prefix[1], s_prefix[2], s_prefix[3]} + wave_prefix,
                              tile, wave, lane);
         if (!has_next) break;
-#pragma unroll
-        for (int k = 0; k < VEC; ++k) a[k] = b[k];
+        for (int k = 0; k < VEC; ++k) a[k] = b[k];  // the prefetched tile becomes the current one
         tile = next;
     }
 }

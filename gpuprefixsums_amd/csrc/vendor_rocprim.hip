@@ -92,7 +92,7 @@ int gps_vendor_time_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, 
             if (e != hipSuccess) status = hip_fail(e, "hipEventElapsedTime");
         }
     }
-    for (auto& ev_i : ev) hipEventDestroy(ev_i);
+    for (auto& ev_i : ev) (void)hipEventDestroy(ev_i);
     return status;
 }
 

@@ -80,7 +80,10 @@ const char* gps_last_error(void);
 int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** out);
 /* (...Dispatcher.cuh:158-163) */
 int gps_destroy(gps_handle* h);
-/* Effective tuning and tile size (elements per workgroup) of a handle. */
+/* Effective tuning and tile size (elements per workgroup) of a handle. A
+ * GPS_ALGO_CSDL handle created without a tile shape or schedule picks both
+ * per call from n (one workgroup per tile, and smaller tiles, below 2^24
+ * elements); for it this reports the configuration used from 2^24 up. */
 int gps_get_tuning(const gps_handle* h, gps_tuning* out, uint32_t* tile_elems);
 
 /* uint32 scans, wraparound mod 2^32, bit-exact with the serial loop of

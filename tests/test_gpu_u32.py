@@ -167,6 +167,20 @@ def test_all_max_values_wrap(name):
         assert np.array_equal(got, ((np.uint64(1 << 32) - k) & np.uint64(0xFFFFFFFF)).astype(np.uint32))
 
 
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_default_handle_across_size_thresholds(inclusive):
+    # A default CSDL handle picks shape and schedule from n: one workgroup per
+    # tile at 1024x2 up to 2^18, 1024x4 up to 2^21, 1024x8 below 2^24, the
+    # persistent 1024x8 kernel from 2^24. Sizes on both sides of each switch.
+    sizes = [1, 5, (1 << 18) - 1, 1 << 18, (1 << 18) + 1, (1 << 21) - 3, 1 << 21, (1 << 21) + 5,
+             (1 << 24) - 1, 1 << 24, (1 << 24) + 7]
+    with gps.ScanHandle(gps.ALGO_CSDL, sizes[-1]) as h:
+        assert (h.block_threads, h.vec_per_thread, h.schedule) == (1024, 8, gps.SCHEDULE_PERSISTENT)
+        for n in sizes:
+            check(h, oracle.gen_u32(n, n, 32), inclusive, f"default handle n={n}")
+        assert h.device_error() == 0
+
+
 def test_argument_errors():
     from gpuprefixsums_amd._lib import GPS_ERR_INVALID, GPS_ERR_TOO_LARGE
     with gps.ScanHandle(gps.ALGO_CSDL, 1000) as h:
