@@ -131,7 +131,8 @@ int gps_graph_destroy(gps_graph* g);
  * takes a tile when all four agree. Replaces the reference's struct scan
  * (GPUPrefixSumsWebGPUapis/SharedShaders/csdldf_struct.wgsl:60-300, a vec4
  * scanned with per-member status words). Needs a GPS_ALGO_CSDL handle with
- * max_n >= n. */
+ * max_n >= n; a default handle picks shape and schedule from n as for the
+ * plain scan. */
 int gps_inclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 int gps_exclusive_scan_u32x4(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, void* stream);
 

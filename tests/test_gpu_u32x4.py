@@ -157,6 +157,17 @@ def test_large_2p25_structs():
             assert h.device_error() == 0
 
 
+@pytest.mark.parametrize("inclusive", [True, False], ids=["incl", "excl"])
+def test_default_handle_across_size_thresholds(inclusive):
+    # A default handle runs one workgroup per tile at 1024x4 up to 2^19
+    # structs, at 1024x8 below 2^22, and the persistent kernel from 2^22.
+    sizes = [1, (1 << 19) - 1, 1 << 19, (1 << 19) + 1, (1 << 22) - 1, 1 << 22, (1 << 22) + 3]
+    with gps.ScanHandle(gps.ALGO_CSDL, sizes[-1]) as h:
+        for n in sizes:
+            check(h, structs(n, n), inclusive, f"default handle n={n}")
+        assert h.device_error() == 0
+
+
 def test_n_zero_is_a_no_op():
     with gps.ScanHandle(gps.ALGO_CSDL, 1024) as h:
         d = sentinel_u32(64)
