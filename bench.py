@@ -347,12 +347,14 @@ class Legs:
         out["survey_n2p26"] = res
 
     def tuner(self, out):
-        """Learned tuner: regret of the MLP cost model's choices (inference on
-        the GPU with the library's kernels) on the 128 held-out workloads of the
-        512-workload sweep, against the exhaustively measured best, and the
-        regret of the library default (no model) on the same workloads."""
+        """Learned tuner: regret of the tuner's choices (MLP cost model, inference
+        on the GPU with the library's kernels, gated against the default) on the
+        128 held-out workloads of the 512-workload sweep, against the
+        exhaustively measured best; and, without the model, the regret of the
+        library's per-size default, of its fixed 1024x8 default and of the best
+        static rule on the same workloads."""
         from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
-        from gpuprefixsums_amd.tuner.evaluate import best_static, load_times, regrets
+        from gpuprefixsums_amd.tuner.evaluate import best_static, load_times, regrets, summarize
         from gpuprefixsums_amd.tuner.model import CostModel
         from gpuprefixsums_amd.tuner.space import c5_workloads, split
         times_path = DATA / "c5_times_mi355x.jsonl"
@@ -362,8 +364,9 @@ class Legs:
         times = load_times(times_path)
         train, held = split([w for w in c5_workloads() if w.wid in times])
         static = best_static(train, times)
-        r_mlp, r_def, r_stat, _ = regrets(CostModel.load(DEFAULT_MODEL), held, times, static)
+        r_mlp, r_def, r_stat, rows = regrets(CostModel.load(DEFAULT_MODEL), held, times, static)
         out["tuner"] = {"held_out": len(held), "regret_mlp": r_mlp, "regret_default": r_def,
+                        "regret_fixed_default": summarize([r["fixed_regret"] for r in rows]),
                         "regret_best_static": r_stat, "best_static": static,
                         "source": "tuner/data/c5_times_mi355x.jsonl (exhaustive harness timings on MI355X)"}
 

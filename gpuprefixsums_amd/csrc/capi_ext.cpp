@@ -18,9 +18,31 @@ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 bool seg_persistent(const gps_handle* h) { return h->schedule != GPS_SCHEDULE_TILE; }
 
 // Segmented kernels use the handle's tile shape when it is one of theirs,
-// otherwise 1024 threads x 8 uint4 (32768-element tiles).
-gps::TileShape seg_shape(const gps_handle* h) {
+// otherwise 1024 threads x 8 uint4 (32768-element tiles). A default handle
+// picks the shape from n, from the exhaustive sweep's timings on MI355X
+// (mean time over the best shape, batched u32 workloads): 1024 x 2 below
+// 2^20 elements (1.00-1.14 against 1.4-1.8 for 1024 x 8), 1024 x 4 below
+// 2^22, 1024 x 8 from there (1.00 against 1.08-1.19 for 1024 x 4).
+gps::TileShape seg_shape(const gps_handle* h, uint64_t n) {
+    if (h->size_adaptive)
+        return n < (1ull << 20) ? gps::TileShape{1024, 2} : n < (1ull << 22) ? gps::TileShape{1024, 4}
+                                                                            : gps::TileShape{1024, 8};
     return gps::seg_shape_supported(h->shape, seg_persistent(h)) ? h->shape : gps::TileShape{1024, 8};
+}
+
+// FP kernels likewise: the handle's shape when it is an FP shape, else
+// 1024 x 4. A default handle picks from n (sweep timings and back-to-back
+// launches on MI355X): one FP scan uses 256 x 4 up to 2^18 elements (2^16:
+// 4.8 against 9.0 us for 1024 x 4 ... 1024 x 8), 1024 x 4 up to 2^22 and
+// 1024 x 8 above; the segmented FP kernel uses 256 x 4 below 2^20 and
+// 1024 x 4 above (1.00 against 1.03-1.08 for 1024 x 8).
+gps::TileShape fp_shape(const gps_handle* h, uint64_t n, bool segmented) {
+    if (h->size_adaptive) {
+        if (segmented) return n < (1ull << 20) ? gps::TileShape{256, 4} : gps::TileShape{1024, 4};
+        return n <= (1ull << 18) ? gps::TileShape{256, 4} : n <= (1ull << 22) ? gps::TileShape{1024, 4}
+                                                                               : gps::TileShape{1024, 8};
+    }
+    return gps::fp_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
 }
 
 int check_seg(gps_handle* h, const void* in, const void* out, const void* offsets, uint64_t n) {
@@ -41,7 +63,7 @@ int enqueue_segmented(gps_handle* h, const uint32_t* in, uint32_t* out, const ui
                       uint64_t num_segments, uint64_t n, bool inclusive, hipStream_t s) {
     if (n == 0) return GPS_OK;
     HIP_TRY(gps::before_epoch_launch(h, s));
-    HIP_TRY(gps::launch_segscan_u32(seg_shape(h), inclusive, seg_persistent(h), in, out, offsets, num_segments, n,
+    HIP_TRY(gps::launch_segscan_u32(seg_shape(h, n), inclusive, seg_persistent(h), in, out, offsets, num_segments, n,
                                     aligned16(in) && aligned16(out), h->status, h->ticket, h->err, s));
     return GPS_OK;
 }
@@ -62,7 +84,7 @@ int scan_fp(gps_handle* h, const void* in, float* out, uint64_t n, bool bf16, in
     const uintptr_t ia = reinterpret_cast<uintptr_t>(in), oa = reinterpret_cast<uintptr_t>(out);
     if ((ia & (bf16 ? 1u : 3u)) || (oa & 3u)) return fail(GPS_ERR_INVALID, "misaligned buffer");
     if (n == 0) return GPS_OK;
-    const gps::TileShape s = gps::fp_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
+    const gps::TileShape s = fp_shape(h, n, false);
     const uint32_t vec_ok = ((ia & (bf16 ? 7u : 15u)) == 0) && ((oa & 15u) == 0);
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(gps::before_epoch_launch(h, st));
@@ -87,7 +109,7 @@ int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offset
     if (compensated < gps::kFpPlain || compensated > gps::kFpPair)
         return fail(GPS_ERR_INVALID, "compensated must be 0 (plain) or 1 (two-float pairs)");
     if (n == 0) return GPS_OK;
-    const gps::TileShape s = gps::fp_shape_supported(h->shape) ? h->shape : gps::TileShape{1024, 4};
+    const gps::TileShape s = fp_shape(h, n, true);
     const uint32_t vec_ok = ((ia & (bf16 ? 7u : 15u)) == 0) && ((oa & 15u) == 0);
     hipStream_t st = static_cast<hipStream_t>(stream);
     HIP_TRY(gps::before_epoch_launch(h, st));
@@ -167,7 +189,7 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
     *out = nullptr;
     int rc = check_seg(h, d_in, d_out, d_offsets, n);
     if (rc) return rc;
-    HIP_TRY(gps::prepare_segscan_u32(seg_shape(h), inclusive != 0, seg_persistent(h)));
+    HIP_TRY(gps::prepare_segscan_u32(seg_shape(h, n), inclusive != 0, seg_persistent(h)));
     hipStream_t s = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     gps_graph* g = new (std::nothrow) gps_graph();
@@ -179,7 +201,7 @@ int gps_segmented_graph_create(gps_handle* h, const uint32_t* d_in, uint32_t* d_
     if (e == hipSuccess) {
         // No re-arm check inside a capture: a graph node must not depend on
         // the host-side launch count (gps_reset re-arms explicitly).
-        rc = n ? (gps::launch_segscan_u32(seg_shape(h), inclusive != 0, seg_persistent(h), d_in, d_out, d_offsets,
+        rc = n ? (gps::launch_segscan_u32(seg_shape(h, n), inclusive != 0, seg_persistent(h), d_in, d_out, d_offsets,
                                           num_segments, n, aligned16(d_in) && aligned16(d_out), h->status,
                                           h->ticket, h->err, s) == hipSuccess
                       ? GPS_OK

@@ -17,7 +17,7 @@ from typing import Optional
 
 import numpy as np
 
-from .space import Config, Workload, candidates, default_config, features
+from .space import Config, Workload, candidates, default_config, features, gated_choice
 
 DATA = Path(__file__).resolve().parent / "data"
 DEFAULT_MODEL = DATA / "mlp_mi355x.npz"
@@ -40,10 +40,11 @@ class Tuner:
 
     def choose(self, w: Workload, fallback: str = "exhaustive") -> Config:
         if self.model is not None and self.in_range(w):
-            p = self.predict_ms(w)
-            return min(p, key=p.get)
+            cands = candidates(w.dtype)
+            pred = self.model.predict(np.array([features(w, c) for c in cands]))
+            return gated_choice(cands, pred, default_config(w.dtype, w.segments, w.n), self.model.margin)
         if fallback == "default":
-            return default_config(w.dtype, w.segments)
+            return default_config(w.dtype, w.segments, w.n)
         if self._runner is None:
             from .harness import Runner
             self._runner = Runner(max_n=max(w.n, 1 << 10))

@@ -5,10 +5,16 @@
 Training set: the first 384 workloads of the sweep; held out: the last 128.
 For a held-out workload with exhaustively measured times T(c):
     regret(choice) = T(choice) / min_c T(c) - 1
-reported as mean / median / max over the held-out set, for the MLP's choice
-and for two methods without the model: the library default, and the best
-static rule (per dtype, the one configuration with the lowest mean regret on
-the training workloads).
+reported as mean / median / max over the held-out set, for the tuner's
+choice and for two methods without the model: the library default (a default
+handle's per-size choice), and the best static rule (per dtype, the one
+configuration with the lowest mean regret on the training workloads).
+
+The tuner takes the model's fastest predicted configuration only when it
+beats the default's predicted time by more than a margin. The margin and the
+training hyper-parameters (weight decay, epochs) are chosen by 4-fold cross-
+validation on the training workloads alone; the held-out workloads judge
+the result once.
 """
 from __future__ import annotations
 
@@ -23,7 +29,13 @@ import numpy as np
 
 from . import DATA, DEFAULT_MODEL
 from .model import CostModel
-from .space import Config, Workload, c5_workloads, candidates, default_config, features, split
+from .space import (Config, Workload, c5_workloads, candidates, default_config, features, fixed_default_config,
+                    gated_choice, split)
+
+CV_FOLDS = 4
+CV_WEIGHT_DECAY = (1e-5, 1e-4, 1e-3)
+CV_EPOCHS = (2000, 4000)
+CV_MARGINS = (0.0, 0.02, 0.05, 0.1, 0.2)
 
 
 def load_times(path: Path) -> Dict[int, Dict[str, float]]:
@@ -57,30 +69,64 @@ def best_static(ws: List[Workload], times: Dict[int, Dict[str, float]]) -> Dict[
 
 
 def regrets(model: CostModel, ws: List[Workload], times: Dict[int, Dict[str, float]], static: Dict[str, str]):
+    """Regrets of the tuner's policy (the gated choice at the model's margin),
+    of the library default and of the static rule over `ws`."""
+    margin = float(getattr(model, "margin", 0.0))
     mlp, dflt, stat, rows = [], [], [], []
     for w in ws:
         t = times[w.wid]
         best = min(t.values())
         cands = [c for c in candidates(w.dtype) if c.name in t]
         pred = model.predict(np.array([features(w, c) for c in cands]))
-        pick = cands[int(np.argmin(pred))]
-        d = default_config(w.dtype, w.segments)
+        d = default_config(w.dtype, w.segments, w.n)
+        pick = gated_choice(cands, pred, d, margin)
         s = static.get(w.dtype, d.name)
         mlp.append(t[pick.name] / best - 1.0)
         dflt.append(t[d.name] / best - 1.0)
         stat.append(t[s] / best - 1.0)
+        fx = fixed_default_config(w.dtype, w.segments).name
         rows.append({"wid": w.wid, "n": w.n, "dtype": w.dtype, "segments": w.segments, "best": min(t, key=t.get),
                      "mlp": pick.name, "default": d.name, "static": s, "mlp_regret": mlp[-1],
-                     "default_regret": dflt[-1], "static_regret": stat[-1]})
-    summ = lambda r: {"mean": float(np.mean(r)), "median": float(np.median(r)), "max": float(np.max(r)),  # noqa: E731
-                      "within_5pct": float(np.mean(np.array(r) <= 0.05))}
-    return summ(mlp), summ(dflt), summ(stat), rows
+                     "default_regret": dflt[-1], "static_regret": stat[-1], "fixed_regret": t[fx] / best - 1.0})
+    return summarize(mlp), summarize(dflt), summarize(stat), rows
+
+
+def summarize(r) -> dict:
+    return {"mean": float(np.mean(r)), "median": float(np.median(r)), "max": float(np.max(r)),
+            "within_5pct": float(np.mean(np.array(r) <= 0.05))}
+
+
+def cross_validate(train: List[Workload], times: Dict[int, Dict[str, float]], seed: int = 0) -> dict:
+    """Chooses (weight decay, epochs, margin) by CV_FOLDS-fold cross-validation
+    on the training workloads: mean regret of the gated choice on each
+    validation fold, averaged over folds; ties go to the smaller margin."""
+    folds = [[w for i, w in enumerate(train) if i % CV_FOLDS == f] for f in range(CV_FOLDS)]
+    scores = {}
+    for wd in CV_WEIGHT_DECAY:
+        for ep in CV_EPOCHS:
+            per_margin = {m: [] for m in CV_MARGINS}
+            for f in range(CV_FOLDS):
+                fit_ws = [w for g in range(CV_FOLDS) if g != f for w in folds[g]]
+                X, y = dataset(fit_ws, times)
+                model = CostModel().fit(X, y, epochs=ep, weight_decay=wd, seed=seed)
+                for w in folds[f]:
+                    t = times[w.wid]
+                    best = min(t.values())
+                    cands = [c for c in candidates(w.dtype) if c.name in t]
+                    pred = model.predict(np.array([features(w, c) for c in cands]))
+                    d = default_config(w.dtype, w.segments, w.n)
+                    for m in CV_MARGINS:
+                        per_margin[m].append(t[gated_choice(cands, pred, d, m).name] / best - 1.0)
+            for m in CV_MARGINS:
+                scores[(wd, ep, m)] = float(np.mean(per_margin[m]))
+    wd, ep, m = min(scores, key=lambda k: (round(scores[k], 6), k[2]))
+    return {"weight_decay": wd, "epochs": ep, "margin": m, "cv_regret": scores[(wd, ep, m)],
+            "grid": {f"wd={k[0]:g} epochs={k[1]} margin={k[2]:g}": v for k, v in scores.items()}}
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--times", default=str(DATA / "c5_times_mi355x.jsonl"))
-    ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save", action="store_true", help="write the model and the regret table")
@@ -92,16 +138,25 @@ def main(argv=None) -> int:
     if not held:
         print("no held-out workloads measured", file=sys.stderr)
         return 1
+    cv = cross_validate(train, times, seed=args.seed)
+    print(f"cross-validation: weight decay {cv['weight_decay']:g}, epochs {cv['epochs']}, "
+          f"margin {cv['margin']:g}, regret {cv['cv_regret']:.4f}", file=sys.stderr)
     X, y = dataset(train, times)
-    model = CostModel().fit(X, y, epochs=args.epochs, lr=args.lr, seed=args.seed)
+    model = CostModel().fit(X, y, epochs=cv["epochs"], lr=args.lr, weight_decay=cv["weight_decay"], seed=args.seed)
+    model.margin = cv["margin"]
     Xh, yh = dataset(held, times)
     rmse_train = float(np.sqrt(np.mean((model.predict(X) - y) ** 2)))
     rmse_held = float(np.sqrt(np.mean((model.predict(Xh) - yh) ** 2)))
     static = best_static(train, times)
     r_mlp, r_def, r_stat, rows = regrets(model, held, times, static)
+    model.margin, margin = 0.0, model.margin
+    r_argmin = regrets(model, held, times, static)[0]
+    model.margin = margin
     res = {"train_workloads": len(train), "held_out_workloads": len(held), "train_samples": int(len(X)),
+           "cross_validation": cv, "regret_mlp_argmin": r_argmin,
            "rmse_log2_train": rmse_train, "rmse_log2_held": rmse_held, "final_loss": float(model.curve[-1]),
            "regret_mlp": r_mlp, "regret_default": r_def, "regret_best_static": r_stat,
+           "regret_fixed_default": summarize([r["fixed_regret"] for r in rows]),
            "best_static": static}
     print(json.dumps(res, indent=1))
     if args.save:

@@ -77,6 +77,7 @@ class CostModel:
 
     def __init__(self, hidden: int = 64):
         self.hidden = hidden
+        self.margin = 0.0  # the tuner's gate: deviate from the default only above this predicted gain
         self.mean: Optional[np.ndarray] = None
         self.std: Optional[np.ndarray] = None
         self.params: Optional[np.ndarray] = None
@@ -120,7 +121,7 @@ class CostModel:
 
     def save(self, path: Path) -> None:
         np.savez(path, params=self.params, mean=self.mean, std=self.std, hidden=self.hidden,
-                 curve=self.curve if self.curve is not None else np.zeros(0))
+                 curve=self.curve if self.curve is not None else np.zeros(0), margin=self.margin)
 
     @classmethod
     def load(cls, path: Path) -> "CostModel":
@@ -128,4 +129,5 @@ class CostModel:
         m = cls(int(d["hidden"]))
         m.params, m.mean, m.std = d["params"], d["mean"], d["std"]
         m.curve = d["curve"]
+        m.margin = float(d["margin"]) if "margin" in d.files else 0.0
         return m

@@ -61,13 +61,43 @@ def candidates(dtype: str) -> List[Config]:
     return list(U32_CONFIGS) if dtype == "u32" else list(FP_CONFIGS)
 
 
-def default_config(dtype: str, segments: int) -> Config:
-    """What the library does without the tuner: the default CSDL shape for a
-    single scan, the default batched kernel for several segments, and for
-    floating point the default FP shape (one launch, plain or segmented)."""
+def default_config(dtype: str, segments: int, n: int = 1 << 30) -> Config:
+    """What the library does without the tuner: a default handle's choice for
+    n elements (the same rules as libgps's size-adaptive defaults). A single
+    u32 scan maps to the persistent 1024x8 CSDL (the default handle's choice
+    from 2^24; below that it runs one workgroup per tile, which has no
+    candidate here)."""
     if dtype != "u32":
-        return Config("loop_fp", 1024, 8) if segments == 1 else Config("batched_fp", 1024, 8)
-    return Config("loop_csdl", 1024, 8) if segments == 1 else Config("batched", 1024, 8)
+        if segments == 1:
+            b, v = (256, 4) if n <= 1 << 18 else (1024, 4) if n <= 1 << 22 else (1024, 8)
+            return Config("loop_fp", b, v)
+        b, v = (256, 4) if n < 1 << 20 else (1024, 4)
+        return Config("batched_fp", b, v)
+    if segments == 1:
+        return Config("loop_csdl", 1024, 8)
+    b, v = (1024, 2) if n < 1 << 20 else (1024, 4) if n < 1 << 22 else (1024, 8)
+    return Config("batched", b, v)
+
+
+def fixed_default_config(dtype: str, segments: int) -> Config:
+    """The library's fixed configuration for every size (what a handle did
+    before its defaults depended on n): 1024x8 throughout."""
+    if dtype != "u32":
+        return Config("loop_fp" if segments == 1 else "batched_fp", 1024, 8)
+    return Config("loop_csdl" if segments == 1 else "batched", 1024, 8)
+
+
+def gated_choice(cands: List[Config], pred_log2: "np.ndarray", default: Config, margin: float) -> Config:
+    """The tuner's policy: the model's fastest predicted configuration if it
+    beats the predicted time of the library default by more than `margin`
+    (relative), else the default. margin = 0 is the plain argmin."""
+    names = [c.name for c in cands]
+    best = int(np.argmin(pred_log2))
+    if default.name not in names:
+        return cands[best]
+    if pred_log2[best] < pred_log2[names.index(default.name)] - math.log2(1.0 + margin):
+        return cands[best]
+    return default
 
 
 @dataclass

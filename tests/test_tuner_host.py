@@ -8,7 +8,7 @@ import pytest
 
 from gpuprefixsums_amd.tuner.evaluate import best_static, regrets
 from gpuprefixsums_amd.tuner.space import (FEATURES, KINDS, Config, Workload, c5_workloads, candidates, default_config,
-                                           features, lognormal_partition, split)
+                                           features, gated_choice, lognormal_partition, split)
 
 
 def test_sweep_shape_and_split():
@@ -41,10 +41,18 @@ def test_config_names_round_trip_and_candidates():
         assert len({c.name for c in cs}) == len(cs)
         for c in cs:
             assert Config.parse(c.name) == c
-        assert default_config(dt, 1).name in {c.name for c in cs}
-        assert default_config(dt, 100).name in {c.name for c in cs}
+        for n in (1 << 10, 1 << 18, (1 << 18) + 1, 1 << 20, 1 << 22, (1 << 22) + 1, 1 << 30):
+            assert default_config(dt, 1, n).name in {c.name for c in cs}
+            assert default_config(dt, 100, n).name in {c.name for c in cs}
     assert {c.kind for c in candidates("f32")} == {"loop_fp", "batched_fp"}
     assert default_config("bf16", 100).kind == "batched_fp" and default_config("u32", 100).kind == "batched"
+    # the library's size-adaptive defaults
+    assert [default_config("u32", 9, n).name for n in (1 << 19, 1 << 21, 1 << 22)] == [
+        "batched:1024x2", "batched:1024x4", "batched:1024x8"]
+    assert [default_config("f32", 1, n).name for n in (1 << 18, 1 << 22, (1 << 22) + 1)] == [
+        "loop_fp:256x4", "loop_fp:1024x4", "loop_fp:1024x8"]
+    assert [default_config("bf16", 9, n).name for n in ((1 << 20) - 1, 1 << 20)] == [
+        "batched_fp:256x4", "batched_fp:1024x4"]
 
 
 def test_features_dimension_and_values():
@@ -77,10 +85,12 @@ class _StubModel:
 
 
 def test_regret_arithmetic_against_hand_computed_values():
-    # single-segment f32 workloads timed on the three one-launch loop_fp shapes
-    ws = [Workload(i, 4096, "f32", 1, np.array([0, 4096], dtype=np.uint64)) for i in range(4)]
+    # single-segment f32 workloads of 2^24 elements (the default handle's
+    # choice there is 1024x8) timed on the three one-launch loop_fp shapes
+    n = 1 << 24
+    ws = [Workload(i, n, "f32", 1, np.array([0, n], dtype=np.uint64)) for i in range(4)]
     names = [c.name for c in candidates("f32") if c.kind == "loop_fp"]  # 1024x8, 1024x4, 256x4
-    assert default_config("f32", 1).name == names[0]
+    assert default_config("f32", 1, n).name == names[0]
     times = {
         0: {names[0]: 2.0, names[1]: 1.0, names[2]: 4.0},
         1: {names[0]: 3.0, names[1]: 1.5, names[2]: 1.0},
@@ -99,3 +109,13 @@ def test_regret_arithmetic_against_hand_computed_values():
     # static 1024x4: 1 and 0
     assert r_stat["mean"] == pytest.approx(0.5)
     assert [r["mlp"] for r in rows] == [names[2], names[2]]
+
+
+def test_gated_choice_policy():
+    cs = [Config("batched", 1024, 8), Config("batched", 1024, 4), Config("batched", 256, 4)]
+    pred = np.log2(np.array([10.0, 9.0, 12.0]))  # 1024x4 predicted 10% faster than the default 1024x8
+    d = cs[0]
+    assert gated_choice(cs, pred, d, 0.0) == cs[1]
+    assert gated_choice(cs, pred, d, 0.05) == cs[1]   # 11% predicted gain > 5%
+    assert gated_choice(cs, pred, d, 0.2) == d        # < 20%: keep the default
+    assert gated_choice(cs[1:], pred[1:], d, 0.2) == cs[1]  # default not a candidate: plain argmin
