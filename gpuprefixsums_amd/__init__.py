@@ -115,6 +115,14 @@ class ScanHandle:
         n = _numel(d_in) if n is None else n
         check(load().gps_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive), _stream(stream)))
 
+    def scan_carry(self, d_in, d_out, n: int, inclusive: bool = True, d_carry_in=None, d_carry_out=None,
+                   stream=None) -> None:
+        """Scan continued from a device carry word (d_carry_in, may be None);
+        d_carry_out (may be None, may equal d_carry_in) receives the total plus
+        the carry."""
+        check(load().gps_scan_u32_carry(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive),
+                                        _ptr(d_carry_in), _ptr(d_carry_out), _stream(stream)))
+
     def time(self, d_in, d_out, n: int, inclusive: bool = True, iters: int = 10, stream=None) -> list:
         ms = (ctypes.c_float * iters)()
         check(load().gps_time_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive),

@@ -62,6 +62,7 @@ SIGNATURES = {
     "gps_inclusive_scan_u32": (_int, [_vp, _vp, _vp, _u64, _vp]),
     "gps_exclusive_scan_u32": (_int, [_vp, _vp, _vp, _u64, _vp]),
     "gps_scan_u32": (_int, [_vp, _vp, _vp, _u64, _int, _vp]),
+    "gps_scan_u32_carry": (_int, [_vp, _vp, _vp, _u64, _int, _vp, _vp, _vp]),
     "gps_segmented_scan_u32": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, _vp]),
     "gps_segmented_scan_u32_per_segment": (_int, [_vp, _vp, _vp, ctypes.POINTER(_u64), _u64, _int, _vp]),
     "gps_segmented_graph_create": (_int, [_vp, _vp, _vp, _vp, _u64, _u64, _int, ctypes.POINTER(_vp)]),

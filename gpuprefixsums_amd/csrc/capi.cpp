@@ -119,14 +119,26 @@ int check_scan_args(gps_handle* h, const void* in, const void* out, uint64_t n) 
     return GPS_OK;
 }
 
-int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, void* stream) {
+int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, void* stream,
+             const uint32_t* carry_in = nullptr, uint32_t* carry_out = nullptr) {
     int rc = check_scan_args(h, in, out, n);
     if (rc) return rc;
-    if (n == 0) return GPS_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n == 0) {
+        if (carry_out) {
+            if (carry_in)
+                HIP_TRY(hipMemcpyAsync(carry_out, carry_in, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+            else
+                HIP_TRY(hipMemsetAsync(carry_out, 0, sizeof(uint32_t), s));
+        }
+        return GPS_OK;
+    }
     gps::TileShape shape = h->shape;
     uint32_t schedule = h->schedule;
     if (h->size_adaptive) size_adaptive_config(n, shape, schedule);
+    if ((carry_in || carry_out) &&
+        (h->algo != GPS_ALGO_CSDL || schedule == GPS_SCHEDULE_CONTROL))
+        return fail(GPS_ERR_UNSUPPORTED, "carries are offered by the CSDL tile and persistent schedules");
     const uint32_t tile = kernel_tile(shape, schedule);
     const uint32_t tiles = (uint32_t)((n + tile - 1) / tile);
     const uint32_t vec_ok = aligned16(in) && aligned16(out);
@@ -147,6 +159,8 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
     p.spin_limit = h->spin_limit;
     p.stall_period = h->stall_period;
     p.counters = h->counters;
+    p.carry_in = carry_in;
+    p.carry_out = carry_out;
     if (h->algo == GPS_ALGO_CSDLDF) {
         HIP_TRY(gps::launch_csdldf_u32(shape, inclusive, p, s));
     } else if (schedule == GPS_SCHEDULE_PERSISTENT) {
@@ -254,6 +268,13 @@ int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out,
 
 int gps_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream) {
     return scan_u32(h, d_in, d_out, n, inclusive != 0, stream);
+}
+
+int gps_scan_u32_carry(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive,
+                       const uint32_t* d_carry_in, uint32_t* d_carry_out, void* stream) {
+    if ((reinterpret_cast<uintptr_t>(d_carry_in) | reinterpret_cast<uintptr_t>(d_carry_out)) & 3u)
+        return fail(GPS_ERR_INVALID, "carry words must be 4-byte aligned");
+    return scan_u32(h, d_in, d_out, n, inclusive != 0, stream, d_carry_in, d_carry_out);
 }
 
 int gps_get_counters(gps_handle* h, uint64_t out[4]) {

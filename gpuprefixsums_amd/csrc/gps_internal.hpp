@@ -34,6 +34,13 @@ struct ScanParams {
     unsigned long long* counters;       // [polls, fallbacks, insertions, stall timeouts] or null
     // persistent CSDL only: +1 per workgroup that takes a tile, or null
     unsigned long long* occupancy;
+    // CSDL (tile and persistent schedules): a device word added to every
+    // output (null: 0), and a device word that receives the scan's total plus
+    // that carry (null: not written). The carry is read before tile 0
+    // publishes and the total written after the last tile's lookback, so the
+    // two may be the same word (chunked scans chain through it).
+    const uint32_t* carry_in;
+    uint32_t* carry_out;
 };
 
 struct TileShape {

@@ -86,8 +86,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES
         const uint32_t lane = lane_id();
         if (f.tile == 0) {
             if (lane == 0) {
-                atomic_store_agent(p.status, make_status(f.tile_total, kFlagIncl, f.epoch));
-                s_prefix = 0;
+                const uint32_t carry = p.carry_in ? *p.carry_in : 0u;
+                atomic_store_agent(p.status, make_status(carry + f.tile_total, kFlagIncl, f.epoch));
+                s_prefix = carry;
+                if (p.carry_out && p.num_tiles == 1) *p.carry_out = carry + f.tile_total;
             }
         } else {
             if (lane == 0) atomic_store_agent(p.status + kSlotWords * f.tile, make_status(f.tile_total, kFlagAgg, f.epoch));
@@ -95,6 +97,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES
             if (lane == 0) {
                 atomic_store_agent(p.status + kSlotWords * f.tile, make_status(prefix + f.tile_total, kFlagIncl, f.epoch));
                 s_prefix = prefix;
+                if (p.carry_out && f.tile == p.num_tiles - 1) *p.carry_out = prefix + f.tile_total;
             }
         }
     }
@@ -128,6 +131,7 @@ csdl_pp_u32(ScanParams p) {
     __shared__ uint32_t s_wave[C::kWaves];
     __shared__ uint32_t s_ticket[3];
     __shared__ uint32_t s_prefix;
+    __shared__ uint32_t s_carry;
     const uint32_t wave = wave_index();
     const uint32_t lane = lane_id();
     const uint32_t total_tickets = p.num_tiles + gridDim.x;
@@ -161,7 +165,12 @@ csdl_pp_u32(ScanParams p) {
             // follows publication order, so lookbacks find contiguous ready
             // runs. (Taking the ticket before the scan, or a whole tile
             // ahead, measured 8% slower at 2^28.)
-            atomic_store_agent(p.status + kSlotWords * tile, make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+            uint32_t v = tile_total;
+            if (tile == 0) {
+                s_carry = p.carry_in ? *p.carry_in : 0u;
+                v += s_carry;
+            }
+            atomic_store_agent(p.status + kSlotWords * tile, make_status(v, tile ? kFlagAgg : kFlagIncl, epoch));
             s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
         }
         __syncthreads();
@@ -181,13 +190,16 @@ csdl_pp_u32(ScanParams p) {
                 load_guarded<VEC>(b, p.in, next_base, p.n);
         }
         if (wave == 0) {
-            uint32_t prefix = 0;
+            uint32_t prefix = tile == 0 ? s_carry : 0u;  // tile 0 starts from the carry-in
             if (tile != 0) {
                 prefix = lb.finish(p.status, epoch, p.err, true);
                 if (lane == 0)
                     atomic_store_agent(p.status + kSlotWords * tile, make_status(prefix + tile_total, kFlagIncl, epoch));
             }
-            if (lane == 0) s_prefix = prefix;
+            if (lane == 0) {
+                s_prefix = prefix;
+                if (p.carry_out && tile == p.num_tiles - 1) *p.carry_out = prefix + tile_total;
+            }
         }
         __syncthreads();
         const uint32_t add = s_prefix + wave_prefix;

@@ -97,6 +97,17 @@ int gps_exclusive_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out,
  * form an FFI binds when it passes the direction as a flag). */
 int gps_scan_u32(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream);
 
+/* The same scan continued from a carry (no reference counterpart; it serves
+ * chunked scans of arrays larger than one call, and the per-shard scan of a
+ * multi-device scan): every output gets *d_carry_in added (mod 2^32), and
+ * *d_carry_out receives the scan's total plus that carry. Both are device
+ * words and either may be NULL (carry 0 / not written); they may be the same
+ * word, so consecutive chunks chain through one word with no host sync.
+ * Offered on GPS_ALGO_CSDL handles with the tile or persistent schedule
+ * (a default handle's every choice); others return GPS_ERR_UNSUPPORTED. */
+int gps_scan_u32_carry(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive,
+                       const uint32_t* d_carry_in, uint32_t* d_carry_out, void* stream);
+
 /* ---- batched segmented scan (no reference counterpart) ------------------- */
 
 /* Scans num_segments segments stored back to back: segment s is

@@ -181,6 +181,60 @@ def test_default_handle_across_size_thresholds(inclusive):
         assert h.device_error() == 0
 
 
+@pytest.mark.parametrize("schedule", ["default", "tile", "persistent"])
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_scan_continued_from_a_carry(schedule, inclusive):
+    sched = {"default": 0, "tile": gps.SCHEDULE_TILE, "persistent": gps.SCHEDULE_PERSISTENT}[schedule]
+    n = (1 << 22) + 123
+    x = oracle.gen_u32(77, n, 32)
+    want = oracle.scan_u32(x, inclusive).astype(np.uint64)
+    total = int(x.astype(np.uint64).sum())
+    C = 0x9E3779B9
+    with gps.ScanHandle(gps.ALGO_CSDL, n, schedule=sched) as h:
+        d_in = to_dev(x)
+        # one call: outputs shifted by the carry, carry_out = total + carry
+        d_out = sentinel_u32(n)
+        c_in = to_dev(np.array([C], np.uint32))
+        c_out = sentinel_u32(1)
+        h.scan_carry(d_in, d_out, n, inclusive, c_in, c_out)
+        sync()
+        got = from_dev_u32(d_out)
+        exp = ((want + C) % (1 << 32)).astype(np.uint32)
+        assert np.array_equal(got, exp), first_mismatch(got, exp)
+        assert int(from_dev_u32(c_out)[0]) == (total + C) % (1 << 32)
+        # chunks chained through one carry word (carry_in == carry_out)
+        word = to_dev(np.zeros(1, np.uint32))
+        d_out = sentinel_u32(n)
+        bounds = [0, 1, 4096, 4096 + 65536 + 3, (1 << 20) + 7, (1 << 21), n]
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            h.scan_carry(d_in[a:b], d_out[a:b], b - a, inclusive, word, word)
+        sync()
+        got = from_dev_u32(d_out)
+        exp = want.astype(np.uint32)  # each chunk starts from the running total
+        assert np.array_equal(got, exp), first_mismatch(got, exp)
+        assert int(from_dev_u32(word)[0]) == total % (1 << 32)
+        assert h.device_error() == 0
+
+
+def test_carry_edge_cases_and_errors():
+    with gps.ScanHandle(gps.ALGO_CSDL, 4096) as h:
+        d = sentinel_u32(16)
+        c_in = to_dev(np.array([12345], np.uint32))
+        c_out = sentinel_u32(1)
+        h.scan_carry(d, d, 0, True, c_in, c_out)  # n = 0: carry passes through
+        sync()
+        assert int(from_dev_u32(c_out)[0]) == 12345
+        h.scan_carry(d, d, 0, True, None, c_out)  # no carry-in: 0
+        sync()
+        assert int(from_dev_u32(c_out)[0]) == 0
+    with gps.ScanHandle(gps.ALGO_RTS, 4096) as h:
+        with pytest.raises(gps.GpsError, match="carries"):
+            h.scan_carry(d, d, 16, True, c_in, None)
+    with gps.ScanHandle(gps.ALGO_CSDL, 4096, schedule=gps.SCHEDULE_CONTROL) as h:
+        with pytest.raises(gps.GpsError, match="carries"):
+            h.scan_carry(d, d, 16, True, None, c_out)
+
+
 def test_argument_errors():
     from gpuprefixsums_amd._lib import GPS_ERR_INVALID, GPS_ERR_TOO_LARGE
     with gps.ScanHandle(gps.ALGO_CSDL, 1000) as h:
