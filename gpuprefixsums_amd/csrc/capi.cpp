@@ -34,6 +34,13 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(GPS_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
 }
 
+int check_device(const gps_handle* h) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != h->device)
+        return fail(GPS_ERR_INVALID, "the handle belongs to device %d but device %d is current", h->device, cur);
+    return GPS_OK;
+}
+
 hipError_t before_epoch_launch(gps_handle* h, hipStream_t s) {
     if (h->launches >= kLaunchesBeforeReset) {
         const hipError_t e = launch_reset_status(h->status, (uint64_t)h->max_tiles * kSlotWords, h->ticket, s);
@@ -110,6 +117,7 @@ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 
 int check_scan_args(gps_handle* h, const void* in, const void* out, uint64_t n) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer with n=%llu", (unsigned long long)n);
     if (n > h->max_n)
         return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
@@ -288,6 +296,7 @@ int gps_get_counters(gps_handle* h, uint64_t out[4]) {
 
 int gps_reset_counters(gps_handle* h) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemset(h->counters, 0, 4 * sizeof(unsigned long long)));
     HIP_TRY(hipDeviceSynchronize());
@@ -305,6 +314,7 @@ int gps_device_error(gps_handle* h, uint32_t* bits) {
 
 int gps_reset(gps_handle* h, void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     HIP_TRY(gps::launch_reset_status(h->status, (uint64_t)h->max_tiles * gps::kSlotWords, h->ticket,
                                      static_cast<hipStream_t>(stream)));
     h->launches = 0;

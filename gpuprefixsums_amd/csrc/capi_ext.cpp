@@ -47,6 +47,7 @@ gps::TileShape fp_shape(const gps_handle* h, uint64_t n, bool segmented) {
 
 int check_seg(gps_handle* h, const void* in, const void* out, const void* offsets, uint64_t n) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "segmented scans need a CSDL handle");
     if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
     if (!offsets) return fail(GPS_ERR_INVALID, "null offsets");
@@ -74,6 +75,7 @@ namespace {
 
 int scan_fp(gps_handle* h, const void* in, float* out, uint64_t n, bool bf16, int compensated, void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "floating-point scans need a CSDL handle");
     if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
     if (n > h->max_n)
@@ -95,6 +97,7 @@ int scan_fp(gps_handle* h, const void* in, float* out, uint64_t n, bool bf16, in
 int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offsets, uint64_t num_segments,
                uint64_t n, bool bf16, int compensated, void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "floating-point scans need a CSDL handle");
     if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
     if (!offsets) return fail(GPS_ERR_INVALID, "null offsets");
@@ -120,6 +123,7 @@ int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offset
 
 int scan_u32x4(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "struct scans need a CSDL handle");
     if (n && (!in || !out)) return fail(GPS_ERR_INVALID, "null data pointer");
     if (n > h->max_n)
@@ -171,6 +175,7 @@ int gps_segmented_scan_u32_per_segment(gps_handle* h, const uint32_t* d_in, uint
                                        const uint64_t* h_offsets, uint64_t num_segments, int inclusive,
                                        void* stream) {
     if (!h) return fail(GPS_ERR_INVALID, "null handle");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (!h_offsets) return fail(GPS_ERR_INVALID, "null offsets");
     for (uint64_t s = 0; s < num_segments; ++s) {
         const uint64_t a = h_offsets[s], b = h_offsets[s + 1];
@@ -268,6 +273,7 @@ int gps_survey_scan_u32(int alg, const uint32_t* d_in, uint32_t* d_out, uint64_t
 int gps_estimate_occupancy(gps_handle* h, const uint32_t* d_in, uint32_t* d_out, uint64_t n, int struct_scan,
                            void* stream, uint32_t* measured, uint32_t* resident) {
     if (!h || !measured || !resident) return fail(GPS_ERR_INVALID, "null argument");
+    if (int rc_ = gps::check_device(h)) return rc_;
     if (h->algo != GPS_ALGO_CSDL) return fail(GPS_ERR_UNSUPPORTED, "the occupancy estimate needs a CSDL handle");
     if (!d_in || !d_out) return fail(GPS_ERR_INVALID, "null data pointer");
     if (n == 0) return fail(GPS_ERR_INVALID, "n must be positive");

@@ -36,6 +36,9 @@ namespace gps {
 // Records the message for gps_last_error() and returns `code`.
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
+// GPS_OK when the handle's device is the current one, else a GPS_ERR_INVALID
+// failure (its buffers live on that device; a launch elsewhere would fault).
+int check_device(const gps_handle* h);
 // Re-arms the status words when the 30-bit epoch would wrap; counts a launch.
 hipError_t before_epoch_launch(gps_handle* h, hipStream_t s);
 
