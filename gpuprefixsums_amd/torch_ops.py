@@ -59,22 +59,23 @@ def cumsum(x: torch.Tensor, inclusive: bool = True, compensated: bool = True) ->
     _check(x, "cumsum")
     x = x.contiguous()
     n = x.numel()
-    stream = torch.cuda.current_stream(x.device)
-    if x.dtype in _INT:
-        out = torch.empty_like(x)
-        if n:
-            _handle(x, n).scan(x, out, n, inclusive, stream=stream)
-        return out
-    if not inclusive:
+    if x.dtype not in _INT and not inclusive:
         raise RuntimeError("gps::cumsum: floating-point scans are inclusive")
-    out = torch.empty(n, dtype=torch.float32, device=x.device)
-    if n:
-        h = _handle(x, n)
-        if x.dtype == torch.bfloat16:
-            h.inclusive_bf16_f32(x.view(torch.int16), out, n, compensated, stream=stream)
-        else:
-            h.inclusive_f32(x, out, n, compensated, stream=stream)
-    return out
+    with torch.cuda.device(x.device):  # the library launches on the current device
+        stream = torch.cuda.current_stream(x.device)
+        if x.dtype in _INT:
+            out = torch.empty_like(x)
+            if n:
+                _handle(x, n).scan(x, out, n, inclusive, stream=stream)
+            return out
+        out = torch.empty(n, dtype=torch.float32, device=x.device)
+        if n:
+            h = _handle(x, n)
+            if x.dtype == torch.bfloat16:
+                h.inclusive_bf16_f32(x.view(torch.int16), out, n, compensated, stream=stream)
+            else:
+                h.inclusive_f32(x, out, n, compensated, stream=stream)
+        return out
 
 
 @cumsum.register_fake
@@ -95,21 +96,22 @@ def segmented_cumsum(x: torch.Tensor, offsets: torch.Tensor, inclusive: bool = F
     n, segments = x.numel(), offsets.numel() - 1
     if segments < 0:
         raise RuntimeError("gps::segmented_cumsum: offsets needs at least one entry")
-    stream = torch.cuda.current_stream(x.device)
-    if x.dtype in _INT:
-        out = torch.empty_like(x)
-        if n:
-            _handle(x, n).segmented(x, out, offsets, segments, n, inclusive, stream=stream)
-        return out
-    if not inclusive:
+    if x.dtype not in _INT and not inclusive:
         raise RuntimeError("gps::segmented_cumsum: floating-point scans are inclusive")
-    out = torch.empty(n, dtype=torch.float32, device=x.device)
-    if n:
-        h = _handle(x, n)
-        src = x.view(torch.int16) if x.dtype == torch.bfloat16 else x
-        h.segmented_fp(src, out, offsets, segments, n, bf16=x.dtype == torch.bfloat16, compensated=compensated,
-                       stream=stream)
-    return out
+    with torch.cuda.device(x.device):  # the library launches on the current device
+        stream = torch.cuda.current_stream(x.device)
+        if x.dtype in _INT:
+            out = torch.empty_like(x)
+            if n:
+                _handle(x, n).segmented(x, out, offsets, segments, n, inclusive, stream=stream)
+            return out
+        out = torch.empty(n, dtype=torch.float32, device=x.device)
+        if n:
+            h = _handle(x, n)
+            src = x.view(torch.int16) if x.dtype == torch.bfloat16 else x
+            h.segmented_fp(src, out, offsets, segments, n, bf16=x.dtype == torch.bfloat16,
+                           compensated=compensated, stream=stream)
+        return out
 
 
 @segmented_cumsum.register_fake
