@@ -1,7 +1,7 @@
 """Build recipe for libgps.so (gfx950) and the CPU oracle.
 
 Compiles every HIP/C++ source under csrc/ with hipcc for gfx950 into
-gpuprefixsums_amd/libgps.so, the reference-style driver gpuprefixsums_amd/gps_main,
+gpuprefixsums_amd/libgps.so, the reference-style test program gpuprefixsums_amd/gps_main,
 and the oracle (oracle/liboracle.so, test infrastructure) with make.
 
     python -m gpuprefixsums_amd.build          # build everything

@@ -1,4 +1,4 @@
-// gps_main: the reference's test-and-time driver (GPUPrefixSumsCUDA/GPUPrefixSumsCUDA.cu:14-38)
+// gps_main: the reference's test-and-time program (GPUPrefixSumsCUDA/GPUPrefixSumsCUDA.cu:14-38)
 // on the MI355X library, through the C++ dispatcher classes.
 //
 //   gps_main [log2_max_size] [batch]     defaults 28 and 100

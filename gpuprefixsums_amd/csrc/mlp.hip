@@ -8,7 +8,7 @@
 // plain one-thread-per-output loop (deterministic summation order, no
 // atomics); matrix cores would not pay at this size.
 //
-// No reference counterpart (a north-star extension). Compiled with
+// No reference counterpart (an extension beside the reference's scans). Compiled with
 // -ffp-contract=off so results are reproducible against the CPU oracle.
 #include <hip/hip_runtime.h>
 

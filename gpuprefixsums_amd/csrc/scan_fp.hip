@@ -1,7 +1,7 @@
 // Floating-point inclusive scans (FP32 or BF16 input, FP32 output) by chained
 // scan with decoupled lookback, with optional two-sum compensated carries.
 //
-// No reference counterpart (a north-star extension); the chained-scan
+// No reference counterpart (an extension beside the reference's scans); the chained-scan
 // structure is the uint32 one (GPUPrefixSumsCUDA/ChainedScanDecoupledLookback.cuh:79-153).
 //
 // Compensated mode carries every partial sum above the element level as an

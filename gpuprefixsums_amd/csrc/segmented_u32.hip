@@ -1,6 +1,6 @@
 // Batched segmented uint32 scan in one launch over the concatenated segments.
 //
-// No reference counterpart (a north-star extension). The nearest reference
+// No reference counterpart (an extension beside the reference's scans). The nearest reference
 // ideas are the persistent ticket loop (GPUPrefixSumsWebGPUapis/SharedShaders/
 // TestVariants/csdldf_occ.wgsl:82-292) and per-member lookback
 // (SharedShaders/csdldf_struct.wgsl:153-291).
@@ -316,7 +316,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // Shapes of the one-workgroup-per-tile kernel, and of the persistent one,
 // which holds two tiles per thread within 128 VGPRs at 4 waves per SIMD. At
 // 1024 x 8 the exclusive form spills 2 VGPRs and still measured 11% faster at
-// C3 than 1024 x 4 (0.080 against 0.090 ms).
+// 4096 log-normal segments of 50.9 M elements than 1024 x 4 (0.080 against
+// 0.090 ms).
 #define GPS_SEG_SHAPES(X) \
     X(256, 4)             \
     X(512, 4)             \

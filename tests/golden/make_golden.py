@@ -21,7 +21,7 @@ import oracle  # noqa: E402
 
 def main() -> None:
     out = {}
-    # uint32 scans: C1-style values in [0, 255], full-range (wrapping) values,
+    # uint32 scans: values in [0, 255], full-range (wrapping) values,
     # and the all-ones known answer, across one to a few tiles.
     for name, seed, n, bits in (("u32_small_values", 11, 9_001, 8), ("u32_full_range", 12, 20_003, 32)):
         x = oracle.gen_u32(seed, n, bits)

@@ -1,7 +1,7 @@
 """Use patterns beyond one scan at a time: a plain scan captured in a HIP
 graph (through torch.cuda.graph) and replayed with eager work in between,
 two handles scanning concurrently on two streams, and the reference-style
-driver executable (the reference's main(), GPUPrefixSumsCUDA.cu:14-38)."""
+test program (the reference's main(), GPUPrefixSumsCUDA.cu:14-38)."""
 import subprocess
 from pathlib import Path
 
@@ -82,7 +82,7 @@ def test_two_handles_scan_concurrently_on_two_streams():
         assert h0.device_error() == 0 and h1.device_error() == 0
 
 
-def test_reference_style_driver_runs_and_passes():
+def test_reference_style_main_runs_and_passes():
     exe = ROOT / "gpuprefixsums_amd" / "gps_main"
     if not exe.exists():
         pytest.fail("gps_main is not built (python -m gpuprefixsums_amd.build)")
