@@ -11,7 +11,7 @@ progress goes to stderr. The CPU baseline is the serial host scan (the
 reference's validator loop restated in C) timed on this machine's host.
 
 Extra legs (reported under "extras"): csdl_exclusive, rts, csdldf (stall
-periods 0/7/8), copy (the read-n + write-n ceiling), vendor (rocPRIM's device
+periods 0/7/8, one workgroup per tile and persistent), copy (the read-n + write-n ceiling), vendor (rocPRIM's device
 scan, the reference's CUB comparator on ROCm), n2p20 (latency at 2^20),
 segmented (4096 log-normal segments: one graph-launched pass vs one launch
 per segment), segmented_fp (the same segments over FP32: one segmented FP
@@ -95,17 +95,20 @@ class Legs:
 
     def csdldf(self, out):
         gps = self.gps
-        for period in (0, 7, 8):
-            with gps.ScanHandle(gps.ALGO_CSDLDF, self.n, stall_period=period) as h:
-                for _ in range(3):
-                    h.inclusive(self.d_in, self.d_out, self.n)
-                h.reset_counters()
-                m = self._timed(h, iters=20)
-                c = h.counters()
-                launches = 23  # 3 warm-up + 20 timed since the reset
-                out[f"csdldf_stall{period}_gbps"] = round(gbps(self.n, m), 1)
-                out[f"csdldf_stall{period}_per_launch"] = {k: round(v / launches, 1) for k, v in c.items()}
-                out[f"csdldf_stall{period}_tiles"] = -(-self.n // h.tile)
+        schedules = (("csdldf", gps.SCHEDULE_TILE, {}),
+                     ("csdldf_pp", gps.SCHEDULE_PERSISTENT, dict(block_threads=1024, vec_per_thread=8)))
+        for name, schedule, shape in schedules:
+            for period in (0, 7, 8):
+                with gps.ScanHandle(gps.ALGO_CSDLDF, self.n, stall_period=period, schedule=schedule, **shape) as h:
+                    for _ in range(3):
+                        h.inclusive(self.d_in, self.d_out, self.n)
+                    h.reset_counters()
+                    m = self._timed(h, iters=20)
+                    c = h.counters()
+                    launches = 23  # 3 warm-up + 20 timed since the reset
+                    out[f"{name}_stall{period}_gbps"] = round(gbps(self.n, m), 1)
+                    out[f"{name}_stall{period}_per_launch"] = {k: round(v / launches, 1) for k, v in c.items()}
+                    out[f"{name}_stall{period}_tiles"] = -(-self.n // h.tile)
 
     def copy(self, out):
         gps, torch = self.gps, self.torch

@@ -66,6 +66,8 @@ uint32_t pp_resident_workgroups(TileShape s, bool inclusive);
 uint32_t cw_tile_elems(TileShape s);
 hipError_t launch_csdl_cw_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
+// Persistent decoupled-fallback scan (grid = resident workgroups).
+hipError_t launch_csdldf_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 hipError_t launch_rts_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,
                           uint32_t num_tiles, uint32_t vec_ok, uint32_t* tile_sums, hipStream_t stream);
 

@@ -380,6 +380,16 @@ csdl_cw_u32(ScanParams p) {
 // never saw j's output.
 // ---------------------------------------------------------------------------
 
+// Polls a waiter makes on one stale word before it reduces that tile itself:
+// spin_limit for the tile's nearest successor and spin_limit more for each
+// tile in between (at most 8 x spin_limit). Every waiter still falls back
+// within a bounded number of polls, but the nearest one usually inserts
+// first, so the waiters behind it do not all re-read the same tile.
+__device__ __forceinline__ uint32_t fallback_patience(uint32_t spin_limit, uint32_t tile, int64_t top) {
+    const int64_t gap = (int64_t)tile - 1 - top;
+    return spin_limit * (1u + (uint32_t)(gap < 7 ? gap : 7));
+}
+
 template <int BLOCK, int VEC, bool INCL>
 __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
     using C = TileCfg<BLOCK, VEC>;
@@ -443,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                         prefix += wave_sum(lane < run ? (uint32_t)w : 0u);
                         top -= run;
                         spins = 0;
-                    } else if (++spins >= p.spin_limit) {
+                    } else if (++spins >= fallback_patience(p.spin_limit, tile, top)) {
                         stale = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
                         break;
                     } else {
@@ -528,6 +538,249 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
     }
     __syncthreads();
     f.store(p, s_prefix + f.wave_prefix);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent, software-pipelined CSDLDF (GPS_SCHEDULE_PERSISTENT).
+//
+// csdl_pp_u32's ticket stream and prefetch (tile i's lookback overlaps tile
+// i+1's loads) with csdldf_u32's lookback: bounded spins, a workgroup-wide
+// redundant reduction of the blocking tile and CAS insertion.
+//
+// Emulated deadlock: a stalled tile's workgroup holds back its publication
+// (and so its next ticket) until a successor has inserted its aggregate. The
+// successor tile is then held by another workgroup, which takes its ticket,
+// publishes and looks back without waiting on anyone, so it reaches the
+// stalled tile, gives up after spin_limit polls and inserts. stall_period >= 2
+// (host-checked) keeps that successor itself unstalled.
+//
+// In-place safety: as in csdldf_u32, every publication is a returning
+// exchange whose result is checked (so waited on) before the barrier that
+// precedes the workgroup's output stores.
+// ---------------------------------------------------------------------------
+// Inclusive publication of tile > 0 by its owner: the word it replaces is
+// this launch's aggregate (the owner's own or a fallback's insertion); any
+// other word is a protocol fault. Checking it also makes the owner wait for
+// the exchange before its output stores.
+__device__ __forceinline__ void publish_incl_checked(const ScanParams& p, uint32_t tile, uint32_t value,
+                                                     uint32_t epoch) {
+    const uint64_t old = atomic_xchg_agent(p.status + kSlotWords * tile, make_status(value, kFlagIncl, epoch));
+    if (status_flag(old, epoch) != (uint32_t)kFlagAgg) raise_device_error(p.err, kErrLookbackWatchdog);
+}
+
+template <int BLOCK, int VEC, bool INCL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(VEC >= 8 ? 4 : GPS_WAVES_PER_EU))) void
+csdldf_pp_u32(ScanParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    __shared__ uint32_t s_wave[C::kWaves];
+    __shared__ uint32_t s_red[C::kWaves];
+    __shared__ uint32_t s_ticket[3];
+    __shared__ uint32_t s_prefix;
+    // Separate flags for the poll and the CAS step: a fast wave 0 may write
+    // the next step's flag while slower waves still read this one.
+    __shared__ uint32_t s_poll_done, s_cas_done, s_fb_tile;
+    __shared__ unsigned long long s_count[4];
+    const uint32_t wave = wave_index();
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, total_tickets);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    __syncthreads();
+    uint32_t tile = s_ticket[0];
+    const uint32_t epoch = s_ticket[1];
+    if (tile >= p.num_tiles) return;
+    if (threadIdx.x == 0) atomic_add_counter(p.occupancy, 1ull);
+
+    // Per-workgroup fallback statistics (polls, fallbacks, insertions, stall
+    // timeouts), kept in LDS by lane 0 of wave 0 and added to the counters
+    // once at the end.
+    if (threadIdx.x < 4) s_count[threadIdx.x] = 0;
+
+    u32x4 a[VEC], b[VEC];
+    {
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
+            load_vec<VEC>(a, p.in + base);
+        else
+            load_guarded<VEC>(a, p.in, base, p.n);
+    }
+
+    while (true) {
+        const uint32_t run = wave_scan_run<VEC, INCL>(a);
+        if (lane == 0) s_wave[wave] = run;
+        __syncthreads();
+        uint32_t wave_prefix, tile_total;
+        combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
+        if (threadIdx.x == 0) {
+            if (p.stall_period && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
+                uint32_t polls = 0;
+                while (status_flag(atomic_load_agent(p.status + kSlotWords * tile), epoch) == 0u) {
+                    if (watchdog_expired(polls, p.err)) {
+                        ++s_count[3];
+                        raise_device_error(p.err, kErrStallWatchdog);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            // Only this workgroup writes an inclusive word for a tile > 0.
+            const uint64_t old = atomic_xchg_agent(p.status + kSlotWords * tile,
+                                                   make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+            if (tile != 0 && status_flag(old, epoch) == (uint32_t)kFlagIncl)
+                raise_device_error(p.err, kErrLookbackWatchdog);
+            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+        }
+        __syncthreads();
+        const uint32_t next = s_ticket[2];
+        const bool has_next = next < p.num_tiles;
+
+        // Lookback state (wave 0; uniform there). The first window is issued
+        // ahead of the prefetch so the in-order vmcnt does not make its use
+        // wait for tile i+1's loads.
+        uint32_t prefix = 0, spins = 0;
+        int32_t top = (int32_t)tile - 1;  // 32-bit: keeps 1024 x 4 within 64 VGPRs
+        uint64_t w = 0, stale = 0;
+        if (wave == 0 && tile != 0) {
+            const int32_t idx = top - (int32_t)lane;
+            const uint64_t v = atomic_load_agent(p.status + kSlotWords * (uint64_t)(idx >= 0 ? idx : 0));
+            w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
+        }
+        if (has_next) {
+            const uint64_t next_base = thread_base<BLOCK, VEC>(next);
+            if (p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n))
+                load_vec<VEC>(b, p.in + next_base);
+            else
+                load_guarded<VEC>(b, p.in, next_base, p.n);
+        }
+
+        while (true) {
+            if (wave == 0) {
+                uint32_t done = tile == 0 ? 1u : 0u;
+                while (!done) {
+                    const uint32_t fl = status_flag(w, epoch);
+                    const uint64_t ready = ballot(fl != 0u);
+                    const uint64_t incl = ballot(fl == (uint32_t)kFlagIncl);
+                    const uint32_t run_len = (~ready) ? (uint32_t)__builtin_ctzll(~ready) : 64u;
+                    const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+                    if (lane == 0) ++s_count[0];
+                    if (first_incl < run_len) {
+                        prefix += wave_sum(lane <= first_incl ? (uint32_t)w : 0u);
+                        done = 1;
+                        break;
+                    }
+                    if (run_len) {
+                        prefix += wave_sum(lane < run_len ? (uint32_t)w : 0u);
+                        top -= run_len;
+                        spins = 0;
+                    } else if (++spins >= fallback_patience(p.spin_limit, tile, top)) {
+                        stale = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
+                        break;
+                    } else {
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    const int32_t idx = top - (int32_t)lane;
+                    const uint64_t v = atomic_load_agent(p.status + kSlotWords * (uint64_t)(idx >= 0 ? idx : 0));
+                    w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
+                }
+                if (lane == 0) {
+                    if (done) {
+                        if (tile != 0) publish_incl_checked(p, tile, prefix + tile_total, epoch);
+                        s_prefix = prefix;
+                    } else {
+                        s_fb_tile = (uint32_t)top;
+                    }
+                    s_poll_done = done;
+                }
+            }
+            __syncthreads();
+            if (s_poll_done) break;
+
+            // Workgroup-wide redundant reduction of tile j (a full tile: only
+            // the last tile is partial and it is nobody's predecessor), FB
+            // vectors per round so that it fits beside the two live tiles.
+            const uint32_t j = s_fb_tile;
+            {
+                constexpr int FB = VEC < 2 ? VEC : 2;
+                const uint64_t fbase = thread_base<BLOCK, VEC>(j);
+                uint32_t tsum = 0;
+#pragma unroll 1
+                for (int c = 0; c < VEC; c += FB) {
+                    u32x4 fd[FB];
+                    if (p.vec_ok)
+                        load_vec<FB>(fd, p.in + fbase + (uint64_t)c * 256u);
+                    else
+                        load_guarded<FB>(fd, p.in, fbase + (uint64_t)c * 256u, p.n);
+                    tsum += thread_sum<FB>(fd);
+                }
+                const uint32_t ws = wave_sum(tsum);
+                if (lane == 0) s_red[wave] = ws;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t fsum = wave_sum(lane < (uint32_t)C::kWaves ? s_red[lane] : 0u);
+                const uint64_t mine = make_status(fsum, j ? kFlagAgg : kFlagIncl, epoch);
+                uint64_t found = mine;
+                if (lane == 0) found = atomic_cas_agent(p.status + kSlotWords * j, stale, mine);
+                found = ((uint64_t)readlane((uint32_t)(found >> 32), 0) << 32) | readlane((uint32_t)found, 0);
+                if (lane == 0) ++s_count[1];
+                uint32_t fl;
+                if (found == stale) {  // inserted ours
+                    if (lane == 0) ++s_count[2];
+                    fl = j ? (uint32_t)kFlagAgg : (uint32_t)kFlagIncl;
+                    found = mine;
+                } else {
+                    fl = status_flag(found, epoch);
+                }
+                uint32_t done = 0;
+                if (fl != 0u) {
+                    prefix += (uint32_t)found;
+                    if (fl == (uint32_t)kFlagIncl)
+                        done = 1;
+                    else
+                        top = (int32_t)j - 1;
+                } else {
+                    // A lost CAS returns a current-epoch word (the only
+                    // in-launch writes); anything else is a protocol fault.
+                    if (lane == 0) raise_device_error(p.err, kErrLookbackWatchdog);
+                    done = 1;
+                }
+                spins = 0;
+                if (lane == 0) {
+                    if (done) {
+                        publish_incl_checked(p, tile, prefix + tile_total, epoch);
+                        s_prefix = prefix;
+                    }
+                    s_cas_done = done;
+                }
+                if (!done) {  // re-issue the window below j
+                    const int32_t idx = top - (int32_t)lane;
+                    const uint64_t v = atomic_load_agent(p.status + kSlotWords * (uint64_t)(idx >= 0 ? idx : 0));
+                    w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
+                }
+            }
+            __syncthreads();
+            if (s_cas_done) break;
+        }
+
+        const uint32_t add = s_prefix + wave_prefix;
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
+            store_vec<VEC>(p.out + base, a, add);
+        else
+            store_guarded<VEC>(p.out, a, add, base, p.n);
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) a[k] = b[k];
+        tile = next;
+    }
+    if (threadIdx.x == 0 && p.counters) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) atomic_add_counter(p.counters + q, s_count[q]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -747,21 +1000,25 @@ hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hip
 
 // Persistent grid: as many workgroups as fit on the device at once (the
 // occupancy query is made once per kernel instance and process).
-template <int B, int V, bool I>
-static uint32_t pp_residency() {
+template <auto KERNEL, int B>
+static uint32_t resident_workgroups() {
     struct Residency {
         uint32_t workgroups = 0;
         Residency() {
             int dev = 0, cus = 0, per_cu = 0;
             if (hipGetDevice(&dev) != hipSuccess) return;
             if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, csdl_pp_u32<B, V, I>, B, 0) != hipSuccess)
-                return;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KERNEL, B, 0) != hipSuccess) return;
             workgroups = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
         }
     };
     static const Residency r;
     return r.workgroups;
+}
+
+template <int B, int V, bool I>
+static uint32_t pp_residency() {
+    return resident_workgroups<csdl_pp_u32<B, V, I>, B>();
 }
 
 template <int B, int V, bool I>
@@ -834,6 +1091,25 @@ hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, h
         else                                                                               \
             hipLaunchKernelGGL((csdldf_u32<B, V, false>), dim3(p.num_tiles), dim3(B), 0, stream, p); \
         return hipGetLastError();                                                          \
+    }
+    GPS_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <int B, int V, bool I>
+static hipError_t launch_df_pp(const ScanParams& p, hipStream_t stream) {
+    const uint32_t resident = resident_workgroups<csdldf_pp_u32<B, V, I>, B>();
+    if (resident == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    hipLaunchKernelGGL((csdldf_pp_u32<B, V, I>), dim3(grid), dim3(B), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_csdldf_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+#define X(B, V)                                                                                       \
+    if (s.block == B && s.vec == V) {                                                                 \
+        return inclusive ? launch_df_pp<B, V, true>(p, stream) : launch_df_pp<B, V, false>(p, stream); \
     }
     GPS_SHAPES(X)
 #undef X

@@ -51,15 +51,18 @@ typedef enum gps_algo {
 typedef struct gps_tuning {
     uint32_t block_threads;  /* 256, 512 or 1024 (whole wave64s) */
     uint32_t vec_per_thread; /* uint4 vectors per thread: 2, 4 or 8 (items = 4x) */
-    uint32_t spin_limit;     /* CSDLDF: polls without progress before a fallback (default 4) */
+    uint32_t spin_limit;     /* CSDLDF: polls without progress before a fallback (default 4), for the
+                                blocking tile's nearest successor; a waiter k tiles farther waits
+                                (1 + min(k, 7)) x spin_limit polls */
     uint32_t stall_period;   /* CSDLDF: stall tiles with index % period == 0 (0 = no emulation, >= 2) */
-    uint32_t schedule;       /* CSDL: GPS_SCHEDULE_* (0 = default) */
+    uint32_t schedule;       /* CSDL, CSDLDF: GPS_SCHEDULE_* (0 = default) */
 } gps_tuning;
 
 /* CSDL workgroup schedules. TILE: one workgroup per tile, as the reference
  * launches (ChainedScanDecoupledLookbackDispatcher.cuh:192-194). PERSISTENT:
  * resident workgroups loop over ticketed tiles and load tile i+1 while tile
- * i's lookback resolves. */
+ * i's lookback resolves. CSDLDF handles take TILE (their default) or
+ * PERSISTENT; RTS handles take neither. */
 #define GPS_SCHEDULE_DEFAULT 0
 #define GPS_SCHEDULE_TILE 1
 #define GPS_SCHEDULE_PERSISTENT 2

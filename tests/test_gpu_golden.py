@@ -20,6 +20,7 @@ HANDLES = {
     "csdl_control": dict(algo=gps.ALGO_CSDL, schedule=gps.SCHEDULE_CONTROL),
     "rts": dict(algo=gps.ALGO_RTS),
     "csdldf_stall7": dict(algo=gps.ALGO_CSDLDF, stall_period=7),
+    "csdldf_pp_stall8": dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_PERSISTENT),
 }
 
 

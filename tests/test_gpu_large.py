@@ -38,6 +38,8 @@ CASES = [
     ("rts", dict(algo=gps.ALGO_RTS)),
     ("csdldf_stall7", dict(algo=gps.ALGO_CSDLDF, stall_period=7)),
     ("csdldf_stall8", dict(algo=gps.ALGO_CSDLDF, stall_period=8)),
+    ("csdldf_pp_stall7", dict(algo=gps.ALGO_CSDLDF, stall_period=7, schedule=gps.SCHEDULE_PERSISTENT)),
+    ("csdldf_pp_stall8", dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_PERSISTENT)),
 ]
 
 
@@ -63,7 +65,8 @@ def test_inclusive_2p28_full_range(data, want_incl, name, kw):
 def test_exclusive_2p28_full_range(data):
     x, d_in, d_out = data
     want = oracle.scan_u32(x, False)
-    for kw in (dict(algo=gps.ALGO_CSDL), dict(algo=gps.ALGO_CSDLDF, stall_period=7)):
+    for kw in (dict(algo=gps.ALGO_CSDL), dict(algo=gps.ALGO_CSDLDF, stall_period=7),
+               dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_PERSISTENT)):
         with gps.ScanHandle(max_n=N, **kw) as h:
             h.exclusive(d_in, d_out, N)
             sync()

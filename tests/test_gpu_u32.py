@@ -4,8 +4,8 @@ Covers the reference's own tests (all-ones sweeps over one to two tiles,
 GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:38-114; random
 [0,1023] inputs with seed = n over n in [3072, 6144),
 GPUPrefixSumsD3D12/GPUPrefixSumBase.h:135-161) and what they leave out:
-full-range wraparound, exclusive decoupled fallback, stall periods 7 and 8,
-n % 4 tails, unaligned and in-place buffers, every tile shape, 2^28 elements.
+full-range wraparound, exclusive decoupled fallback, stall periods 7 and 8
+(and 2, 3 on the persistent schedule), n % 4 tails, unaligned and in-place buffers, every tile shape, 2^28 elements.
 """
 import numpy as np
 import pytest
@@ -28,6 +28,10 @@ ALGOS = {
     "csdldf": dict(algo=gps.ALGO_CSDLDF),
     "csdldf_stall7": dict(algo=gps.ALGO_CSDLDF, stall_period=7),
     "csdldf_stall8": dict(algo=gps.ALGO_CSDLDF, stall_period=8),
+    # decoupled fallback on the persistent pipelined schedule
+    "csdldf_pp": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_PERSISTENT),
+    "csdldf_pp_stall7": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_PERSISTENT, stall_period=7),
+    "csdldf_pp_stall8": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_PERSISTENT, stall_period=8),
 }
 
 
@@ -110,7 +114,7 @@ def test_reference_ones_sweep_one_to_two_tiles(name, inclusive):
 
 
 @pytest.mark.parametrize("block,vec", [(b, v) for b in (256, 512, 1024) for v in (2, 4, 8)])
-@pytest.mark.parametrize("name", ["csdl", "csdl_tile", "csdl_control", "rts", "csdldf_stall7"])
+@pytest.mark.parametrize("name", ["csdl", "csdl_tile", "csdl_control", "rts", "csdldf_stall7", "csdldf_pp_stall7"])
 def test_every_tile_shape(block, vec, name):
     with gps.ScanHandle(max_n=1 << 20, block_threads=block, vec_per_thread=vec, **ALGOS[name]) as h:
         T = h.tile
@@ -294,12 +298,14 @@ def test_device_validators():
     assert gps.validate_ones_u32(ones, n, False) == n
 
 
+@pytest.mark.parametrize("schedule", ["tile", "persistent"])
 @pytest.mark.parametrize("period", [7, 8])
 @pytest.mark.parametrize("inclusive", [True, False])
-def test_emulated_deadlock_counters(period, inclusive):
+def test_emulated_deadlock_counters(period, inclusive, schedule):
     n = 1 << 22
     x = oracle.gen_u32(21, n, 32)
-    with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period, spin_limit=4) as h:
+    sched = {"tile": gps.SCHEDULE_TILE, "persistent": gps.SCHEDULE_PERSISTENT}[schedule]
+    with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period, spin_limit=4, schedule=sched) as h:
         h.reset_counters()
         check(h, x, inclusive, f"stall{period}")
         c = h.counters()
@@ -309,6 +315,25 @@ def test_emulated_deadlock_counters(period, inclusive):
         assert c["stall_timeouts"] == 0
         assert c["insertions"] >= stalled
         assert c["fallbacks"] >= c["insertions"]
+        assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("spin_limit", [1, 32])
+@pytest.mark.parametrize("period", [2, 3])
+def test_persistent_fallback_dense_stalls(period, spin_limit):
+    # Every second or third tile stalls on the persistent schedule: each stalled
+    # workgroup holds its next ticket back until a successor has inserted its
+    # aggregate, so the grid keeps going only through fallbacks.
+    n = (1 << 22) + 77
+    x = oracle.gen_u32(period * 100 + spin_limit, n, 32)
+    with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period, spin_limit=spin_limit,
+                        schedule=gps.SCHEDULE_PERSISTENT) as h:
+        for inclusive in (True, False):
+            h.reset_counters()
+            check(h, x, inclusive, f"period {period} spin {spin_limit}")
+            c = h.counters()
+            stalled = len(range(0, -(-n // h.tile) - 1, period))
+            assert c["stall_timeouts"] == 0 and c["insertions"] >= stalled
         assert h.device_error() == 0
 
 

@@ -79,7 +79,9 @@ def tile_program(mem, t, agg, epoch, num_tiles, period, spin_limit, stats, resul
             spins = 0
             continue
         spins += 1
-        if spins < spin_limit:
+        # patience grows with the distance to the blocking tile's nearest
+        # successor (fallback_patience in scan_u32.hip)
+        if spins < spin_limit * (1 + min(t - 1 - top, 7)):
             continue
         # fallback onto tile `top`: reduce its input, CAS over the stale word
         stats["fallback"] += 1
