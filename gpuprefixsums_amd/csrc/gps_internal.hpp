@@ -43,6 +43,24 @@ struct ScanParams {
     uint32_t* carry_out;
 };
 
+// Workgroups of KERNEL (BLOCK threads) resident on the current device at
+// once: the grid of a persistent launch. Queried once per kernel and process.
+template <auto KERNEL, int BLOCK>
+uint32_t resident_workgroups() {
+    struct Residency {
+        uint32_t workgroups = 0;
+        Residency() {
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess) return;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KERNEL, BLOCK, 0) != hipSuccess) return;
+            workgroups = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
+        }
+    };
+    static const Residency r;
+    return r.workgroups;
+}
+
 struct TileShape {
     uint32_t block;
     uint32_t vec;
@@ -85,8 +103,12 @@ hipError_t launch_segscan_u32(TileShape s, bool inclusive, bool persistent, cons
 bool fp_shape_supported(TileShape s);
 // Carries of the FP scans (values match GPS_FP_* in gps.h).
 enum FpCarry : int { kFpPlain = 0, kFpPair = 1, kFpF64 = 2 };
-hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, const void* in, float* out, uint64_t n,
-                          uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
+// persistent: resident workgroups loop over ticketed tiles, loading tile i+1
+// while tile i's lookback resolves (otherwise, and always for FP64 carries,
+// one workgroup per tile).
+hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, const void* in, float* out,
+                          uint64_t n, uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err,
+                          hipStream_t stream);
 // Batched segmented FP inclusive scan (one launch; same shapes as launch_scan_fp).
 hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out,
                              const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,

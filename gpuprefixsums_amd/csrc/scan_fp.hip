@@ -438,6 +438,199 @@ __global__ __launch_bounds__(BLOCK) void scan_fp_f64(FpParams p) {
 }
 
 // ---------------------------------------------------------------------------
+// Persistent, software-pipelined FP scan (the uint32 csdl_pp_u32 schedule):
+// resident workgroups loop over ticketed tiles; with tile i scanned and its
+// aggregate published, a workgroup takes its next ticket and issues tile
+// i+1's loads, then resolves tile i's lookback and stores tile i. The carry
+// arithmetic comes from a policy, PairCarry<COMP> (pair or plain floats),
+// with the same per-level operations and order as scan_fp.
+//
+// 2^27 FP32 / BF16 elements at 1024 x 8: pairs 4998 / 4433 GB/s against
+// 4787 / 4009 for one workgroup per tile, plain 5567 / 5285 against 5383 /
+// 4921; at 2^23 one workgroup per tile is faster. FP64 carries have no
+// persistent form: with two register tiles they spill at 1024 x 8, and at
+// 1024 x 4 they ran 4722 against 5067 GB/s for scan_fp_f64 at 1024 x 8.
+// ---------------------------------------------------------------------------
+template <bool COMP>
+struct PairCarry {
+    using T = ff;
+    static __device__ __forceinline__ T zero() { return {0.f, 0.f}; }
+    static __device__ __forceinline__ T add(T a, T b) { return ff_add<COMP>(a, b); }
+    static __device__ __forceinline__ T add1(T a, float v) { return ff_add1<COMP>(a, v); }
+    static __device__ __forceinline__ T vec_total(const f32x4& v) {
+        T t = {v.x, 0.f};
+        t = ff_add1<COMP>(t, v.y);
+        t = ff_add1<COMP>(t, v.z);
+        return ff_add1<COMP>(t, v.w);
+    }
+    static __device__ __forceinline__ T scan(T x) { return wave_incl_scan_ff<COMP>(x); }
+    static __device__ __forceinline__ T shr1(T x) {  // wave_shr:1, lane 0 gets zero
+        return {GPS_DPP_F(x.s, 0x138, 0xf, true), COMP ? GPS_DPP_F(x.c, 0x138, 0xf, true) : 0.f};
+    }
+    static __device__ __forceinline__ T lane(T x, int l) { return {readlane_f(x.s, l), COMP ? readlane_f(x.c, l) : 0.f}; }
+    static __device__ __forceinline__ void publish(uint64_t* st, uint32_t tile, T v, uint64_t flag, uint32_t epoch) {
+        publish_ff<COMP>(st, tile, v, flag, epoch);
+    }
+    static __device__ __forceinline__ T from_words(uint64_t w0, uint64_t w1) {
+        return {__builtin_bit_cast(float, (uint32_t)w0), __builtin_bit_cast(float, (uint32_t)w1)};
+    }
+    static __device__ __forceinline__ float round(T a) { return a.s + a.c; }
+};
+
+// Window lookback over two-word tile values, split into issue() and consume()
+// so the first poll can be issued ahead of the next tile's loads.
+template <class Ops>
+struct LookbackW2 {
+    typename Ops::T prefix;
+    int32_t top;
+    uint64_t w0, w1;
+    uint32_t empty_polls;
+
+    __device__ __forceinline__ void start(uint32_t tile) {
+        prefix = Ops::zero();
+        top = (int32_t)tile - 1;
+        empty_polls = 0;
+    }
+    __device__ __forceinline__ void issue(const uint64_t* status, uint32_t epoch) {
+        const int32_t idx = top - (int32_t)lane_id();
+        const uint64_t at = (uint64_t)(idx >= 0 ? idx : 0);
+        const uint64_t l0 = atomic_load_agent(status + kSlotWords * at);
+        const uint64_t l1 = atomic_load_agent(status + kSlotWords * at + 1);
+        const uint64_t sentinel = make_status(0u, kFlagIncl, epoch);
+        w0 = idx >= 0 ? l0 : sentinel;
+        w1 = idx >= 0 ? l1 : sentinel;
+    }
+    // True once `prefix` is complete (or the watchdog fired).
+    __device__ __forceinline__ bool consume(uint32_t epoch, uint32_t* err) {
+        const uint32_t lane = lane_id();
+        const uint32_t f0 = status_flag(w0, epoch), f1 = status_flag(w1, epoch);
+        const uint32_t f = f0 == f1 ? f0 : 0u;
+        const uint64_t ready = ballot(f != 0u);
+        const uint64_t incl = ballot(f == (uint32_t)kFlagIncl);
+        const uint32_t run = (~ready) ? (uint32_t)__builtin_ctzll(~ready) : 64u;
+        const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+        const uint32_t take = first_incl < run ? first_incl + 1u : run;
+        if (take) {
+            const typename Ops::T v = lane < take ? Ops::from_words(w0, w1) : Ops::zero();
+            prefix = Ops::add(Ops::lane(Ops::scan(v), 63), prefix);
+            if (first_incl < run) return true;
+            top -= (int32_t)run;
+            empty_polls = 0;
+        } else {
+            if (watchdog_expired(empty_polls, err)) {
+                if (lane == 0) raise_device_error(err, kErrLookbackWatchdog);
+                return true;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return false;
+    }
+};
+
+template <int BLOCK, int VEC, bool BF16, class Ops>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void scan_fp_pp(
+    FpParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    using T = typename Ops::T;
+    __shared__ T s_w[C::kWaves];
+    __shared__ T s_p;
+    __shared__ uint32_t s_ticket[3];
+    const uint32_t wave = wave_index();
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, total_tickets);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    __syncthreads();
+    uint32_t tile = s_ticket[0];
+    const uint32_t epoch = s_ticket[1];
+    if (tile >= p.num_tiles) return;
+
+    f32x4 x[VEC], y[VEC];
+    load_fp<BF16, VEC>(x, p.in, thread_base<BLOCK, VEC>(tile), p.n, p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n));
+
+    while (true) {
+        // Per vector: lane total, wave exclusive scan -> lane carry.
+        T lane_carry[VEC];
+        T run = Ops::zero();
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const T s = Ops::scan(Ops::vec_total(x[k]));
+            lane_carry[k] = Ops::add(run, Ops::shr1(s));
+            run = Ops::add(run, Ops::lane(s, 63));
+        }
+        if (lane == 0) s_w[wave] = run;
+        __syncthreads();
+        T wave_pre, tile_tot;
+        {
+            const T v = lane < C::kWaves ? s_w[lane] : Ops::zero();
+            const T sc = Ops::scan(v);
+            wave_pre = Ops::lane(Ops::shr1(sc), (int)wave);
+            tile_tot = Ops::lane(sc, C::kWaves - 1);
+        }
+        if (threadIdx.x == 0) {
+            Ops::publish(p.status, tile, tile_tot, tile ? kFlagAgg : kFlagIncl, epoch);
+            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+        }
+        __syncthreads();
+        const uint32_t next = s_ticket[2];
+        const bool has_next = next < p.num_tiles;
+
+        LookbackW2<Ops> lb;
+        if (wave == 0 && tile != 0) {
+            lb.start(tile);
+            lb.issue(p.status, epoch);
+        }
+        if (has_next)
+            load_fp<BF16, VEC>(y, p.in, thread_base<BLOCK, VEC>(next), p.n,
+                               p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n));
+        if (wave == 0) {
+            T prefix = Ops::zero();
+            if (tile != 0) {
+                while (!lb.consume(epoch, p.err)) lb.issue(p.status, epoch);
+                prefix = lb.prefix;
+                if (lane == 0) Ops::publish(p.status, tile, Ops::add(prefix, tile_tot), kFlagIncl, epoch);
+            }
+            if (lane == 0) s_p = prefix;
+        }
+        __syncthreads();
+        const T base_carry = Ops::add(s_p, wave_pre);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            T acc = Ops::add(base_carry, lane_carry[k]);
+            f32x4 o;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                acc = Ops::add1(acc, x[k][c]);
+                o[c] = Ops::round(acc);
+            }
+            x[k] = o;
+        }
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n)) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+                __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                    if (i < p.n) p.out[i] = x[k][c];
+                }
+        }
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) x[k] = y[k];
+        tile = next;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched segmented FP inclusive scan: one launch over concatenated segments
 // (segment s covers [offsets[s], offsets[s+1])), heads found as in the uint32
 // segmented scan (segments.hpp). A floating-point carry cannot be cut at a
@@ -643,8 +836,24 @@ bool fp_shape_supported(TileShape s) {
     return false;
 }
 
-hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, const void* in, float* out, uint64_t n,
-                          uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream) {
+template <int B, int V, bool BF16, class Ops>
+static hipError_t launch_fp_pp(const FpParams& p, hipStream_t stream) {
+    const uint32_t resident = resident_workgroups<scan_fp_pp<B, V, BF16, Ops>, B>();
+    if (resident == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    hipLaunchKernelGGL((scan_fp_pp<B, V, BF16, Ops>), dim3(grid), dim3(B), 0, stream, p);
+    return hipGetLastError();
+}
+
+template <int B, int V, bool BF16>
+static hipError_t launch_fp_pp_carry(int carry, const FpParams& p, hipStream_t stream) {
+    if (carry == kFpPair) return launch_fp_pp<B, V, BF16, PairCarry<true>>(p, stream);
+    return launch_fp_pp<B, V, BF16, PairCarry<false>>(p, stream);
+}
+
+hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, const void* in, float* out,
+                          uint64_t n, uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err,
+                          hipStream_t stream) {
     const bool compensated = carry == kFpPair;
     FpParams p{};
     p.in = in;
@@ -657,6 +866,9 @@ hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, const void* in, flo
     p.err = err;
 #define X(B, V)                                                                                                 \
     if (s.block == B && s.vec == V) {                                                                           \
+        if (persistent && carry != kFpF64)                                                                      \
+            return bf16 ? launch_fp_pp_carry<B, V, true>(carry, p, stream)                                      \
+                        : launch_fp_pp_carry<B, V, false>(carry, p, stream);                                    \
         if (carry == kFpF64 && bf16)                                                                            \
             hipLaunchKernelGGL((scan_fp_f64<B, V, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);         \
         else if (carry == kFpF64)                                                                               \

@@ -998,24 +998,7 @@ hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hip
     return hipErrorInvalidValue;
 }
 
-// Persistent grid: as many workgroups as fit on the device at once (the
-// occupancy query is made once per kernel instance and process).
-template <auto KERNEL, int B>
-static uint32_t resident_workgroups() {
-    struct Residency {
-        uint32_t workgroups = 0;
-        Residency() {
-            int dev = 0, cus = 0, per_cu = 0;
-            if (hipGetDevice(&dev) != hipSuccess) return;
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, KERNEL, B, 0) != hipSuccess) return;
-            workgroups = (uint32_t)(cus > 0 ? cus : 1) * (uint32_t)(per_cu > 0 ? per_cu : 1);
-        }
-    };
-    static const Residency r;
-    return r.workgroups;
-}
-
+// Persistent grid: as many workgroups as fit on the device at once.
 template <int B, int V, bool I>
 static uint32_t pp_residency() {
     return resident_workgroups<csdl_pp_u32<B, V, I>, B>();

@@ -72,14 +72,18 @@ def test_integer_valued_inputs_are_exact(bf16, comp):
     assert np.array_equal(y.astype(np.float64), want)
 
 
+@pytest.mark.parametrize("schedule", ["tile", "persistent"])
 @pytest.mark.parametrize("shape", [(256, 4), (1024, 4), (1024, 8), (0, 0)])
 @pytest.mark.parametrize("comp", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("n", [1, 3, 4095, 4097, 65537, (1 << 22) + 7])
-def test_f32_normal_bounds(n, comp, shape):
+def test_f32_normal_bounds(n, comp, shape, schedule):
+    # One workgroup per tile, or the persistent pipelined kernel (pair and
+    # plain carries; FP64 carries run one workgroup per tile on both).
     x = oracle.gen_f32_normal(4, n)
     y64 = oracle.scan_f32_f64(x)
     S = oracle.abs_scan_f32_f64(x)
-    with gps.ScanHandle(gps.ALGO_CSDL, n, block_threads=shape[0], vec_per_thread=shape[1]) as h:
+    sched = {"tile": gps.SCHEDULE_TILE, "persistent": gps.SCHEDULE_PERSISTENT}[schedule]
+    with gps.ScanHandle(gps.ALGO_CSDL, n, block_threads=shape[0], vec_per_thread=shape[1], schedule=sched) as h:
         tile = h.tile  # the FP kernels run the handle's shape (all shapes here are FP shapes)
         y = run(h, x, False, comp)
         assert h.device_error() == 0
