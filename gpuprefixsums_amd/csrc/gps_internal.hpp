@@ -109,8 +109,9 @@ enum FpCarry : int { kFpPlain = 0, kFpPair = 1, kFpF64 = 2 };
 hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, const void* in, float* out,
                           uint64_t n, uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err,
                           hipStream_t stream);
-// Batched segmented FP inclusive scan (one launch; same shapes as launch_scan_fp).
-hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out,
+// Batched segmented FP inclusive scan (one launch; same shapes as launch_scan_fp;
+// persistent as there).
+hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, bool persistent, const void* in, float* out,
                              const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,
                              uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream);
 

@@ -823,6 +823,179 @@ __global__ __launch_bounds__(BLOCK) void segscan_fp(SegFpParams p) {
     }
 }
 
+// Persistent, software-pipelined form of segscan_fp (the segscan_pp_u32
+// schedule): while wave 0 resolves tile i's lookback, the last wave finds and
+// marks tile i+1's heads into the other half of a double-buffered LDS bitmap
+// and every wave issues tile i+1's loads. The output pass re-reads tile i's
+// head nibbles from its bitmap instead of holding them in registers beside
+// the two register tiles.
+template <int BLOCK, int VEC, bool BF16, bool COMP>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void segscan_fp_pp(SegFpParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    using Ops = PairCarry<COMP>;
+    constexpr uint32_t kWords = C::kTile / 32;
+    constexpr uint32_t kSearchWave = C::kWaves > 1 ? C::kWaves - 1 : 0;
+    __shared__ uint32_t s_heads[2][kWords];
+    __shared__ float s_ws[C::kWaves], s_wc[C::kWaves];
+    __shared__ uint32_t s_wf[C::kWaves];
+    __shared__ uint32_t s_ticket[3];
+    __shared__ float s_ps, s_pc;
+    __shared__ uint64_t s_first_seg;
+    const uint32_t wave = wave_index();
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, total_tickets);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    for (uint32_t i = threadIdx.x; i < kWords; i += BLOCK) s_heads[0][i] = 0u;
+    __syncthreads();
+    uint32_t tile = s_ticket[0];
+    const uint32_t epoch = s_ticket[1];
+    if (tile >= p.num_tiles) return;
+
+    f32x4 x[VEC], y[VEC];
+    load_fp<BF16, VEC>(x, p.in, thread_base<BLOCK, VEC>(tile), p.n, p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n));
+    {
+        const uint64_t lo = (uint64_t)tile * C::kTile;
+        const uint64_t hi = lo + C::kTile < p.n ? lo + C::kTile : p.n;
+        if (wave == 0) {
+            const uint64_t s0 = find_first_segment(p.offsets, p.num_segments, lo);
+            if (lane == 0) s_first_seg = s0;
+        }
+        __syncthreads();
+        mark_heads<BLOCK>(s_heads[0], p.offsets, p.num_segments, s_first_seg, lo, hi, tile == 0);
+        __syncthreads();
+    }
+
+    const uint32_t run_base = wave * C::kWaveElems;
+    uint32_t cur = 0;
+    while (true) {
+        // Scan phase, as segscan_fp: lane-local segmented sums, the segmented
+        // wave scan and each lane's carry-in per vector.
+        ff lane_carry[VEC];
+        uint32_t open = 0;
+        ff run = {0.f, 0.f};
+        uint32_t run_f = 0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const uint32_t nib = head_nibble(s_heads[cur], run_base + (uint32_t)k * 256u + lane * 4u);
+            ff t = {x[k].x, 0.f};
+            t = (nib & 2u) ? ff{x[k].y, 0.f} : ff_add1<COMP>(t, x[k].y);
+            t = (nib & 4u) ? ff{x[k].z, 0.f} : ff_add1<COMP>(t, x[k].z);
+            t = (nib & 8u) ? ff{x[k].w, 0.f} : ff_add1<COMP>(t, x[k].w);
+            const bool any_head = ballot(nib != 0u) != 0ull;
+            uint32_t sf = 0u, ef = 0u;
+            ff sv = t;
+            if (any_head) {
+                sf = nib != 0u;
+                wave_seg_scan_ff<COMP>(sf, sv);
+                ef = GPS_DPP_I(sf, 0x138, 0xf, true);  // wave_shr:1, lane 0 gets 0
+            } else {
+                sv = wave_incl_scan_ff<COMP>(t);
+            }
+            const ff ev = Ops::shr1(sv);
+            lane_carry[k] = ef ? ev : ff_add<COMP>(run, ev);
+            open |= (ef | run_f) ? 0u : (1u << k);
+            const uint32_t tf = readlane(sf, 63);
+            seg_combine_ff<COMP>(run_f, run, tf, Ops::lane(sv, 63));
+        }
+        if (lane == 0) {
+            s_ws[wave] = run.s;
+            s_wc[wave] = run.c;
+            s_wf[wave] = run_f;
+        }
+        __syncthreads();
+        uint32_t pre_f, tile_f;
+        ff pre, tile_tot;
+        {
+            const uint32_t q = lane < C::kWaves ? lane : 0u;
+            uint32_t f = lane < C::kWaves ? s_wf[q] : 0u;
+            ff v = {lane < C::kWaves ? s_ws[q] : 0.f, lane < C::kWaves ? s_wc[q] : 0.f};
+            wave_seg_scan_ff<COMP>(f, v);
+            const uint32_t ef = GPS_DPP_I(f, 0x138, 0xf, true);
+            pre_f = readlane(ef, (int)wave);
+            pre = Ops::lane(Ops::shr1(v), (int)wave);
+            tile_f = readlane(f, C::kWaves - 1);
+            tile_tot = Ops::lane(v, C::kWaves - 1);
+        }
+        const bool need_lookback = tile != 0 && (s_heads[cur][0] & 1u) == 0u;
+        if (threadIdx.x == 0) {
+            // a tile with a head knows its carry-out: publish it inclusive now
+            publish_ff<COMP>(p.status, tile, tile_tot, (tile_f || tile == 0) ? kFlagIncl : kFlagAgg, epoch);
+            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+        }
+        // the other head buffer was last read by the previous tile's output pass
+        for (uint32_t i = threadIdx.x; i < kWords; i += BLOCK) s_heads[cur ^ 1u][i] = 0u;
+        __syncthreads();
+        const uint32_t next = s_ticket[2];
+        const bool has_next = next < p.num_tiles;
+
+        LookbackW2<Ops> lb;
+        if (wave == 0 && need_lookback) {
+            lb.start(tile);
+            lb.issue(p.status, epoch);
+        }
+        if (has_next && wave == kSearchWave) {
+            const uint64_t lo = (uint64_t)next * C::kTile;
+            const uint64_t hi = lo + C::kTile < p.n ? lo + C::kTile : p.n;
+            const uint64_t s0 = find_first_segment(p.offsets, p.num_segments, lo);
+            mark_heads_by(s_heads[cur ^ 1u], p.offsets, p.num_segments, s0, lo, hi, next == 0, lane, kWave);
+        }
+        if (has_next)
+            load_fp<BF16, VEC>(y, p.in, thread_base<BLOCK, VEC>(next), p.n,
+                               p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n));
+        if (wave == 0) {
+            ff prefix = {0.f, 0.f};
+            if (need_lookback) {
+                while (!lb.consume(epoch, p.err)) lb.issue(p.status, epoch);
+                prefix = lb.prefix;
+                if (!tile_f && lane == 0)
+                    publish_ff<COMP>(p.status, tile, ff_add<COMP>(prefix, tile_tot), kFlagIncl, epoch);
+            }
+            if (lane == 0) {
+                s_ps = prefix.s;
+                s_pc = prefix.c;
+            }
+        }
+        __syncthreads();  // also completes the next tile's head bitmap
+        const ff run_in = pre_f ? pre : ff_add<COMP>({s_ps, s_pc}, pre);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const uint32_t nib = head_nibble(s_heads[cur], run_base + (uint32_t)k * 256u + lane * 4u);
+            ff acc = (open >> k) & 1u ? ff_add<COMP>(run_in, lane_carry[k]) : lane_carry[k];
+            f32x4 o;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((nib >> c) & 1u) acc = {0.f, 0.f};
+                acc = ff_add1<COMP>(acc, x[k][c]);
+                o[c] = acc.s + acc.c;
+            }
+            x[k] = o;
+        }
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n)) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+                __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                    if (i < p.n) p.out[i] = x[k][c];
+                }
+        }
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) x[k] = y[k];
+        tile = next;
+        cur ^= 1u;
+    }
+}
+
 #define GPS_FP_SHAPES(X) \
     X(256, 4)            \
     X(1024, 4)           \
@@ -888,7 +1061,16 @@ hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, co
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const void* in, float* out,
+template <int B, int V, bool BF16, bool COMP>
+static hipError_t launch_segfp_pp(const SegFpParams& p, hipStream_t stream) {
+    const uint32_t resident = resident_workgroups<segscan_fp_pp<B, V, BF16, COMP>, B>();
+    if (resident == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    hipLaunchKernelGGL((segscan_fp_pp<B, V, BF16, COMP>), dim3(grid), dim3(B), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, bool persistent, const void* in, float* out,
                              const uint64_t* offsets, uint64_t num_segments, uint64_t n, uint32_t vec_ok,
                              uint64_t* status, uint64_t* ticket, uint32_t* err, hipStream_t stream) {
     SegFpParams p{};
@@ -904,6 +1086,11 @@ hipError_t launch_segscan_fp(TileShape s, bool bf16, bool compensated, const voi
     p.err = err;
 #define X(B, V)                                                                                                  \
     if (s.block == B && s.vec == V) {                                                                            \
+        if (persistent)                                                                                          \
+            return bf16 ? (compensated ? launch_segfp_pp<B, V, true, true>(p, stream)                            \
+                                       : launch_segfp_pp<B, V, true, false>(p, stream))                          \
+                        : (compensated ? launch_segfp_pp<B, V, false, true>(p, stream)                           \
+                                       : launch_segfp_pp<B, V, false, false>(p, stream));                        \
         if (bf16 && compensated)                                                                                 \
             hipLaunchKernelGGL((segscan_fp<B, V, true, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);     \
         else if (bf16)                                                                                           \

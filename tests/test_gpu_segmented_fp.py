@@ -20,6 +20,8 @@ torch = pytest.importorskip("torch")
 import gpuprefixsums_amd as gps  # noqa: E402
 
 SHAPES = [(256, 4), (1024, 4), (1024, 8)]
+# one workgroup per tile, or the persistent pipelined kernel
+SCHEDULES = {"tile": gps.SCHEDULE_TILE, "persistent": gps.SCHEDULE_PERSISTENT}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -64,10 +66,12 @@ def offsets_cases(tile):
     return cases
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("bf16", [False, True])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_integer_valued_inputs_are_exact(shape, bf16):
-    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 22, block_threads=shape[0], vec_per_thread=shape[1]) as h:
+def test_integer_valued_inputs_are_exact(shape, bf16, schedule):
+    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 22, block_threads=shape[0], vec_per_thread=shape[1],
+                        schedule=SCHEDULES[schedule]) as h:
         for name, off in offsets_cases(h.tile).items():
             n = int(off[-1])
             v = (oracle.gen_u32(21 + n, n, 4).astype(np.int64) - 8).astype(np.float32)  # |prefix| < 2^24
@@ -80,10 +84,12 @@ def test_integer_valued_inputs_are_exact(shape, bf16):
         assert h.device_error() == 0
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("comp", [True, False])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_f32_normal_bounds(shape, comp):
-    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 22, block_threads=shape[0], vec_per_thread=shape[1]) as h:
+def test_f32_normal_bounds(shape, comp, schedule):
+    with gps.ScanHandle(gps.ALGO_CSDL, 1 << 22, block_threads=shape[0], vec_per_thread=shape[1],
+                        schedule=SCHEDULES[schedule]) as h:
         for name, off in offsets_cases(h.tile).items():
             n = int(off[-1])
             x = oracle.gen_f32_normal(31 + n, n)
@@ -103,6 +109,26 @@ def test_bf16_logmag_bounds(comp):
         check_bounds(y, oracle.segmented_scan_fp_f64(b, off, bf16=True),
                      oracle.segmented_scan_fp_f64(b, off, bf16=True, absolute=True), comp, h.tile)
         assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_default_handle_on_both_sides_of_2p22(bf16):
+    # A default handle switches FP32 input to the persistent 1024 x 8 kernel
+    # above 2^22 elements; BF16 input stays on one workgroup per tile.
+    for n_target in ((1 << 22) - 1000, (1 << 22) + 40_000):
+        off = oracle.gen_lognormal_offsets(13, 700, median=4096.0, sigma=1.0, lo=1, hi=1 << 20)
+        off = np.minimum(off, n_target).astype(np.uint64)
+        off[-1] = n_target
+        n = n_target
+        if bf16:
+            x = oracle.gen_bf16_logmag(17, n)
+        else:
+            x = oracle.gen_f32_normal(17, n)
+        with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+            y = run(h, x, off, bf16, True)
+            assert h.device_error() == 0
+        check_bounds(y, oracle.segmented_scan_fp_f64(x, off, bf16=bf16),
+                     oracle.segmented_scan_fp_f64(x, off, bf16=bf16, absolute=True), True, 1)
 
 
 def test_one_segment_matches_plain_fp_scan_bounds_and_per_segment_launches():
