@@ -8,8 +8,12 @@ configuration fixes how it runs:
   loop_csdl    one chained-scan launch per segment (u32)       tile shape
   loop_rts     one reduce-then-scan launch per segment (u32)   tile shape
   loop_csdldf  one decoupled-fallback launch per segment (u32) spin limit
-  loop_fp      one compensated FP scan launch per segment      tile shape
-  batched_fp   the one-launch segmented FP kernel              tile shape
+  loop_fp      one compensated FP scan launch per segment      tile shape, schedule
+  batched_fp   the one-launch segmented FP kernel              tile shape, schedule
+
+The FP kinds run one workgroup per tile, or (":pp") the persistent pipelined
+kernel. The u32 kinds run as a handle with that shape runs them: batched and
+loop_csdl persistent, loop_rts and loop_csdldf one workgroup per tile.
 
 Reference knobs these generalise: CSDL_WARPS / UINT4_PER_THREAD
 (GPUPrefixSumsCUDA/ChainedScanDecoupledLookbackDispatcher.cuh:13-14) and
@@ -33,18 +37,20 @@ class Config:
     block: int
     vec: int
     spin: int = 0
+    persistent: bool = False  # FP kinds: the persistent pipelined kernel
 
     @property
     def name(self) -> str:
         s = f"{self.kind}:{self.block}x{self.vec}"
-        return s + (f":spin{self.spin}" if self.kind == "loop_csdldf" else "")
+        s += f":spin{self.spin}" if self.kind == "loop_csdldf" else ""
+        return s + (":pp" if self.persistent else "")
 
     @staticmethod
     def parse(name: str) -> "Config":
         parts = name.split(":")
         b, v = (int(t) for t in parts[1].split("x"))
-        spin = int(parts[2][4:]) if len(parts) > 2 else 0
-        return Config(parts[0], b, v, spin)
+        spin = next((int(t[4:]) for t in parts[2:] if t.startswith("spin")), 0)
+        return Config(parts[0], b, v, spin, "pp" in parts[2:])
 
 
 U32_CONFIGS = (
@@ -54,7 +60,9 @@ U32_CONFIGS = (
     + [Config("loop_csdldf", 1024, 4, s) for s in (1, 4, 32)]
 )
 FP_SHAPES = ((1024, 8), (1024, 4), (256, 4))
-FP_CONFIGS = [Config("loop_fp", b, v) for b, v in FP_SHAPES] + [Config("batched_fp", b, v) for b, v in FP_SHAPES]
+FP_PP_SHAPES = ((1024, 8), (1024, 4))
+FP_CONFIGS = ([Config(k, b, v) for k in ("loop_fp", "batched_fp") for b, v in FP_SHAPES]
+              + [Config(k, b, v, persistent=True) for k in ("loop_fp", "batched_fp") for b, v in FP_PP_SHAPES])
 
 
 def candidates(dtype: str) -> List[Config]:
@@ -70,7 +78,9 @@ def default_config(dtype: str, segments: int, n: int = 1 << 30) -> Config:
     if dtype != "u32":
         if segments == 1:
             b, v = (256, 4) if n <= 1 << 18 else (1024, 4) if n <= 1 << 22 else (1024, 8)
-            return Config("loop_fp", b, v)
+            return Config("loop_fp", b, v, persistent=n > 1 << 24)
+        if dtype == "f32" and n > 1 << 22:
+            return Config("batched_fp", 1024, 8, persistent=True)
         b, v = (256, 4) if n < 1 << 20 else (1024, 4)
         return Config("batched_fp", b, v)
     if segments == 1:
@@ -81,7 +91,8 @@ def default_config(dtype: str, segments: int, n: int = 1 << 30) -> Config:
 
 def fixed_default_config(dtype: str, segments: int) -> Config:
     """The library's fixed configuration for every size (what a handle did
-    before its defaults depended on n): 1024x8 throughout."""
+    before its defaults depended on n): 1024x8 throughout, FP kernels one
+    workgroup per tile."""
     if dtype != "u32":
         return Config("loop_fp" if segments == 1 else "batched_fp", 1024, 8)
     return Config("loop_csdl" if segments == 1 else "batched", 1024, 8)
@@ -146,7 +157,7 @@ def split(workloads: List[Workload], train: int = 384):
 
 
 FEATURES = ["log2_n", "log2_segments", "log2_mean_len", "log2_max_len", "is_u32", "is_f32", "is_bf16"] + [
-    f"kind_{k}" for k in KINDS] + ["log2_block", "log2_vec", "log2_spin"]
+    f"kind_{k}" for k in KINDS] + ["log2_block", "log2_vec", "log2_spin", "persistent"]
 
 
 def workload_features(w: Workload) -> List[float]:
@@ -156,7 +167,8 @@ def workload_features(w: Workload) -> List[float]:
 
 
 def config_features(c: Config) -> List[float]:
-    return [float(c.kind == k) for k in KINDS] + [math.log2(c.block), math.log2(c.vec), math.log2(c.spin + 1.0)]
+    return [float(c.kind == k) for k in KINDS] + [math.log2(c.block), math.log2(c.vec), math.log2(c.spin + 1.0),
+                                                  float(c.persistent)]
 
 
 def features(w: Workload, c: Config) -> List[float]:

@@ -49,20 +49,25 @@ def test_config_names_round_trip_and_candidates():
     # the library's size-adaptive defaults
     assert [default_config("u32", 9, n).name for n in (1 << 19, 1 << 21, 1 << 22)] == [
         "batched:1024x2", "batched:1024x4", "batched:1024x8"]
-    assert [default_config("f32", 1, n).name for n in (1 << 18, 1 << 22, (1 << 22) + 1)] == [
-        "loop_fp:256x4", "loop_fp:1024x4", "loop_fp:1024x8"]
-    assert [default_config("bf16", 9, n).name for n in ((1 << 20) - 1, 1 << 20)] == [
-        "batched_fp:256x4", "batched_fp:1024x4"]
+    assert [default_config("f32", 1, n).name for n in (1 << 18, 1 << 22, (1 << 22) + 1, 1 << 24, (1 << 24) + 1)] == [
+        "loop_fp:256x4", "loop_fp:1024x4", "loop_fp:1024x8", "loop_fp:1024x8", "loop_fp:1024x8:pp"]
+    assert [default_config("bf16", 9, n).name for n in ((1 << 20) - 1, 1 << 20, 1 << 23)] == [
+        "batched_fp:256x4", "batched_fp:1024x4", "batched_fp:1024x4"]
+    assert [default_config("f32", 9, n).name for n in (1 << 22, (1 << 22) + 1)] == [
+        "batched_fp:1024x4", "batched_fp:1024x8:pp"]
+    assert Config.parse("loop_fp:1024x8:pp") == Config("loop_fp", 1024, 8, persistent=True)
 
 
 def test_features_dimension_and_values():
     w = Workload(0, 1 << 20, "bf16", 4, np.array([0, 1, 2, 3, 1 << 20], dtype=np.uint64))
     f = features(w, Config("loop_csdldf", 1024, 4, 32))
-    assert len(f) == len(FEATURES) == 7 + len(KINDS) + 3
+    assert len(f) == len(FEATURES) == 7 + len(KINDS) + 4
     d = dict(zip(FEATURES, f))
     assert d["log2_n"] == 20 and d["log2_segments"] == 2
     assert d["is_bf16"] == 1 and d["is_u32"] == 0
     assert d["kind_loop_csdldf"] == 1 and d["log2_block"] == 10 and d["log2_spin"] == math.log2(33)
+    assert d["persistent"] == 0 and dict(zip(FEATURES, features(w, Config("loop_fp", 1024, 8, persistent=True))))[
+        "persistent"] == 1
 
 
 class _StubModel:
@@ -79,7 +84,8 @@ class _StubModel:
         for row in np.asarray(X):
             kind = [k for k in KINDS if row[col[f"kind_{k}"]] == 1][0]
             c = Config(kind, int(round(2 ** row[col["log2_block"]])), int(round(2 ** row[col["log2_vec"]])),
-                       int(round(2 ** row[col["log2_spin"]] - 1)) if kind == "loop_csdldf" else 0)
+                       int(round(2 ** row[col["log2_spin"]] - 1)) if kind == "loop_csdldf" else 0,
+                       bool(row[col["persistent"]]))
             out.append(self.table.get(c.name, 100.0))
         return np.array(out)
 
@@ -89,7 +95,7 @@ def test_regret_arithmetic_against_hand_computed_values():
     # choice there is 1024x8) timed on the three one-launch loop_fp shapes
     n = 1 << 24
     ws = [Workload(i, n, "f32", 1, np.array([0, n], dtype=np.uint64)) for i in range(4)]
-    names = [c.name for c in candidates("f32") if c.kind == "loop_fp"]  # 1024x8, 1024x4, 256x4
+    names = [c.name for c in candidates("f32") if c.kind == "loop_fp" and not c.persistent]  # 1024x8, 1024x4, 256x4
     assert default_config("f32", 1, n).name == names[0]
     times = {
         0: {names[0]: 2.0, names[1]: 1.0, names[2]: 4.0},
