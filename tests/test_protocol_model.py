@@ -3,9 +3,11 @@ deadlock, epochs), run under random interleavings. No GPU.
 
 The model mirrors the device code step by step (gpuprefixsums_amd/csrc/
 device_common.hpp: status words, tickets, window lookback;
-scan_u32.hip: csdldf_u32): every shared-memory access is one step, and a
-seeded scheduler picks which resident tile steps next. Tiles become resident
-in ticket order, at most `resident` at a time. Several launches reuse one
+scan_u32.hip: csdldf_u32 and csdldf_pp_u32): every shared-memory access is
+one step, and a seeded scheduler picks which resident tile (or persistent
+workgroup) steps next. Tiles become resident in ticket order, at most
+`resident` at a time; persistent workgroups are all resident and loop over
+tickets. Several launches reuse one
 status array, so stale words from earlier epochs are present. The checks:
 each tile's exclusive prefix equals the true prefix mod 2^32, and across the
 runs every fallback branch is taken (timeout, CAS win, CAS lost to an
@@ -42,8 +44,8 @@ class Memory:
         return found
 
 
-def tile_program(mem, t, agg, epoch, num_tiles, period, spin_limit, stats, result):
-    """Generator: one tile of the decoupled-fallback kernel (one yield per access)."""
+def stall_then_publish(mem, t, agg, epoch, num_tiles, period, stats):
+    """Generator: stall emulation, then the tile's aggregate publication."""
     # stall emulation: tiles t % period == 0 that have a successor wait until
     # their own word has been inserted by a successor
     if period and t % period == 0 and t + 1 < num_tiles:
@@ -55,9 +57,11 @@ def tile_program(mem, t, agg, epoch, num_tiles, period, spin_limit, stats, resul
                 break
     mem.status[t] = make_status(agg[t], AGG if t else INCL, epoch)  # exchange
     yield
-    if t == 0:
-        result[t] = 0
-        return
+
+
+def lookback(mem, t, agg, epoch, spin_limit, stats):
+    """Generator returning tile t's exclusive prefix (t > 0): window polls,
+    fallback reductions and CAS insertions, one yield per access."""
     prefix, top, spins = 0, t - 1, 0
     while True:
         # window read: lane l reads tile top - l (one step per word)
@@ -104,9 +108,43 @@ def tile_program(mem, t, agg, epoch, num_tiles, period, spin_limit, stats, resul
         if fl == INCL:
             break
         top = j - 1
+    return prefix & MASK32
+
+
+def resolve(mem, t, agg, epoch, spin_limit, stats, result):
+    """Generator: tile t's lookback and inclusive publication."""
+    if t == 0:
+        result[t] = 0
+        return
+    prefix = yield from lookback(mem, t, agg, epoch, spin_limit, stats)
+    assert flag_of(mem.status[t], epoch) == AGG, "the owner's word must be an aggregate before its inclusive one"
     mem.status[t] = make_status(prefix + agg[t], INCL, epoch)
     yield
-    result[t] = prefix & MASK32
+    result[t] = prefix
+
+
+def tile_program(mem, t, agg, epoch, num_tiles, period, spin_limit, stats, result):
+    """Generator: one tile of the one-workgroup-per-tile decoupled-fallback
+    kernel (csdldf_u32)."""
+    yield from stall_then_publish(mem, t, agg, epoch, num_tiles, period, stats)
+    yield from resolve(mem, t, agg, epoch, spin_limit, stats, result)
+
+
+def workgroup_program(mem, tickets, agg, epoch, num_tiles, period, spin_limit, stats, result):
+    """Generator: one workgroup of the persistent decoupled-fallback kernel
+    (csdldf_pp_u32). It takes its next ticket right after publishing a
+    tile's aggregate, then resolves that tile; a stalled tile therefore holds
+    back the workgroup's next ticket."""
+    t = tickets[0]
+    tickets[0] += 1
+    yield
+    while t < num_tiles:
+        yield from stall_then_publish(mem, t, agg, epoch, num_tiles, period, stats)
+        nxt = tickets[0]
+        tickets[0] += 1
+        yield
+        yield from resolve(mem, t, agg, epoch, spin_limit, stats, result)
+        t = nxt
 
 
 def run_launch(mem, agg, epoch, period, spin_limit, resident, rng, stats):
@@ -126,6 +164,28 @@ def run_launch(mem, agg, epoch, period, spin_limit, resident, rng, stats):
             active.remove(t)
         steps += 1
         assert steps < 5_000_000, "protocol did not terminate"
+    return result
+
+
+def run_persistent(mem, agg, epoch, period, spin_limit, grid, rng, stats):
+    """All `grid` workgroups are resident from the start (the persistent
+    grid is sized to what is resident) and share one ticket counter."""
+    num_tiles = len(agg)
+    result = [None] * num_tiles
+    tickets = [0]
+    progs = {g: workgroup_program(mem, tickets, agg, epoch, num_tiles, period, spin_limit, stats, result)
+             for g in range(min(grid, num_tiles))}
+    active = list(progs)
+    steps = 0
+    while active:
+        g = rng.choice(active)
+        try:
+            next(progs[g])
+        except StopIteration:
+            active.remove(g)
+        steps += 1
+        assert steps < 5_000_000, "protocol did not terminate"
+    assert tickets[0] == num_tiles + min(grid, num_tiles)  # every workgroup ends on one ticket past the tiles
     return result
 
 
@@ -178,3 +238,39 @@ def test_stale_epoch_words_are_never_consumed():
     for t, r in enumerate(res):
         assert r == acc
         acc = (acc + agg[t]) & MASK32
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_persistent_schedule_under_random_interleavings(seed):
+    # The persistent kernel's stalls hold back a workgroup's next ticket; the
+    # launch must still finish (no deadlock) with every prefix exact.
+    rng = random.Random(500 + seed)
+    num_tiles = rng.choice([1, 2, 9, 40, 130])
+    mem = Memory(200)
+    stats = Counter()
+    for launch in range(4):
+        epoch = launch + 1
+        agg = [rng.getrandbits(32) for _ in range(num_tiles)]
+        res = run_persistent(mem, agg, epoch, period=rng.choice([0, 2, 3, 7, 8]), spin_limit=rng.choice([1, 2, 4]),
+                             grid=rng.choice([2, 5, 17]), rng=rng, stats=stats)
+        acc = 0
+        for t in range(num_tiles):
+            assert res[t] == acc, (seed, launch, t)
+            acc = (acc + agg[t]) & MASK32
+        assert all(flag_of(mem.status[t], epoch) == INCL for t in range(num_tiles))
+
+
+def test_persistent_schedule_exercises_the_fallback_branches():
+    stats = Counter()
+    for seed in range(30):
+        rng = random.Random(2000 + seed)
+        mem = Memory(64)
+        agg = [rng.getrandbits(32) for _ in range(48)]
+        res = run_persistent(mem, agg, 1 + seed % 3, period=rng.choice([2, 3, 7, 8]), spin_limit=1,
+                             grid=rng.choice([2, 3, 9]), rng=rng, stats=stats)
+        acc = 0
+        for t, r in enumerate(res):
+            assert r == acc
+            acc = (acc + agg[t]) & MASK32
+    for branch in ("fallback", "cas_win", "cas_lost_agg", "fallback_tile0", "stall_released"):
+        assert stats[branch] > 0, (branch, dict(stats))
