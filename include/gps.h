@@ -199,7 +199,10 @@ int gps_survey_scan_u32(int alg, const uint32_t* d_in, uint32_t* d_out, uint64_t
  *   GPS_FP_F64   (2): one FP64 value, each output rounded once (same
  *                     accuracy as the pair, fewer instructions).
  * Other values return GPS_ERR_INVALID. One kernel launch; needs a
- * GPS_ALGO_CSDL handle. */
+ * GPS_ALGO_CSDL handle. Plain and pair carries run the persistent kernel on a
+ * handle with the persistent schedule (a default handle: above 2^24
+ * elements), else one workgroup per tile; FP64 carries always run one
+ * workgroup per tile. */
 #define GPS_FP_PLAIN 0
 #define GPS_FP_PAIR 1
 #define GPS_FP_F64 2
@@ -239,7 +242,10 @@ int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch
  * segments allowed). Same input/output types as the plain FP scans;
  * compensated is GPS_FP_PLAIN or GPS_FP_PAIR (GPS_FP_F64 returns
  * GPS_ERR_UNSUPPORTED). The carry restarts at every segment head at every
- * level (lane, wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. */
+ * level (lane, wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. Runs the
+ * persistent kernel on a handle with the persistent schedule (a default
+ * handle: FP32 input above 2^22 elements, at 1024 x 8), else one workgroup
+ * per tile. */
 int gps_segmented_scan_f32(gps_handle* h, const float* d_in, float* d_out, const uint64_t* d_offsets,
                            uint64_t num_segments, uint64_t n, int compensated, void* stream);
 int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, const uint64_t* d_offsets,
