@@ -1,7 +1,8 @@
 """n = 2^28 full-range uint32 scans (1 GiB in, 1 GiB out), bit-exact against
 the CPU oracle over every element: CSDL, reduce-then-scan, and the decoupled
 fallback under emulated deadlock with stall periods 7 (every 7th tile) and 8
-(the reference's !(idx & 7), GPUPrefixSumsCUDA/EmulatedDeadlocking.cuh:16,65).
+(the reference's !(idx & 7), GPUPrefixSumsCUDA/EmulatedDeadlocking.cuh:16,65);
+and scans of more than 2^32 elements, checked on the device.
 """
 import numpy as np
 import pytest
@@ -72,3 +73,44 @@ def test_exclusive_2p28_full_range(data):
             sync()
             got = from_dev_u32(d_out)
             assert np.array_equal(got, want), first_mismatch(got, want)
+
+
+def test_beyond_2p32_elements():
+    # n = 2^32 + 4099 uint32 (16 GiB in, 16 GiB out): every index, tile base
+    # and status slot above 32 bits. Checked on the device: the validator
+    # compares every difference scan[i] - scan[i-1] with the input and the
+    # first element with its closed form, which fixes the whole scan; a
+    # planted error above 2^32 shows that the check reaches that far.
+    n = (1 << 32) + 4099
+    d_in = torch.empty(n, dtype=torch.int32, device="cuda")
+    gps.fill_random_u32(d_in, n, seed=32, bits=32)
+    d_out = torch.empty(n, dtype=torch.int32, device="cuda")
+    cases = [dict(algo=gps.ALGO_CSDL), dict(algo=gps.ALGO_CSDL, schedule=gps.SCHEDULE_TILE), dict(algo=gps.ALGO_RTS),
+             dict(algo=gps.ALGO_CSDLDF, stall_period=8)]
+    for kw in cases:
+        with gps.ScanHandle(max_n=n, **kw) as h:
+            for inclusive in (True, False):
+                h.scan(d_in, d_out, n, inclusive)
+                sync()
+                assert gps.validate_scan_u32(d_in, d_out, n, inclusive) == 0, (kw, inclusive)
+                assert h.device_error() == 0
+    # the batched segmented scan with heads below and above 2^32: each segment
+    # checked as a plain scan of its own range
+    off = np.array([0, 1 << 31, (1 << 32) + 100, n], dtype=np.uint64)
+    d_off = torch.from_numpy(off.view(np.int64)).cuda()
+    with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+        for inclusive in (True, False):
+            h.segmented(d_in, d_out, d_off, off.size - 1, n, inclusive)
+            sync()
+            for a, b in zip(off[:-1].tolist(), off[1:].tolist()):
+                assert gps.validate_scan_u32(d_in[a:b], d_out[a:b], b - a, inclusive) == 0, (a, b, inclusive)
+            assert h.device_error() == 0
+    # d_out holds the segmented exclusive scan: plant the error inside the
+    # segment [2^31, 2^32 + 100), above 2^32
+    k = (1 << 32) + 5
+    d_out[k] += 1
+    sync()
+    a, b = 1 << 31, (1 << 32) + 100
+    assert gps.validate_scan_u32(d_in[a:b], d_out[a:b], b - a, False) == 2
+    del d_in, d_out
+    torch.cuda.empty_cache()
