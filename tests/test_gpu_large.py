@@ -114,3 +114,27 @@ def test_beyond_2p32_elements():
     assert gps.validate_scan_u32(d_in[a:b], d_out[a:b], b - a, False) == 2
     del d_in, d_out
     torch.cuda.empty_cache()
+
+
+def test_fp_beyond_2p32_elements():
+    # n = 2^32 + 4099 FP32 ones: with pair or FP64 carries every output is the
+    # exact prefix i + 1 rounded once to FP32 (round to nearest even), so it
+    # must equal float32(i + 1) bit for bit, above 2^32 too.
+    n = (1 << 32) + 4099
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty(n, dtype=torch.float32, device="cuda")
+    with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+        for comp in (gps.FP_PAIR, gps.FP_F64):
+            y.fill_(-1.0)
+            h.inclusive_f32(x, y, n, comp)
+            sync()
+            assert h.device_error() == 0
+            step = 1 << 28
+            for a in range(0, n, step):
+                b = min(a + step, n)
+                want = (torch.arange(a, b, dtype=torch.float64, device="cuda") + 1.0).to(torch.float32)
+                bad = torch.nonzero(y[a:b] != want)
+                assert bad.numel() == 0, (comp, a + int(bad[0]), float(y[a + int(bad[0])]), float(want[int(bad[0])]))
+                del want
+    del x, y
+    torch.cuda.empty_cache()
