@@ -105,13 +105,12 @@ int scan_fp(gps_handle* h, const void* in, float* out, uint64_t n, bool bf16, in
 }
 
 // The segmented FP scan runs its persistent kernel on a handle with the
-// persistent schedule. A default handle runs it, at 1024 x 8, for FP32 input
-// above 2^22 elements (4096 log-normal segments of 50.9 M elements: pairs
-// 0.1067 against 0.1158 ms for one workgroup per tile at 1024 x 4, plain
-// 0.0862 against 0.1023 ms) and keeps one workgroup per tile for BF16 input
-// (pairs 0.1033 against 0.1070 ms).
-bool segfp_persistent(const gps_handle* h, uint64_t n, bool bf16) {
-    if (h->size_adaptive) return !bf16 && n > (1ull << 22);
+// persistent schedule. A default handle runs it, at 1024 x 8, above 2^22
+// elements (4096 log-normal segments of 50.9 M elements: FP32 pairs 0.1021
+// against 0.1145 ms for one workgroup per tile at 1024 x 4, plain 0.0804
+// against 0.1030 ms, BF16 pairs 0.0999 against 0.1032 ms).
+bool segfp_persistent(const gps_handle* h, uint64_t n) {
+    if (h->size_adaptive) return n > (1ull << 22);
     return h->schedule == GPS_SCHEDULE_PERSISTENT;
 }
 
@@ -133,7 +132,7 @@ int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offset
     if (compensated < gps::kFpPlain || compensated > gps::kFpPair)
         return fail(GPS_ERR_INVALID, "compensated must be 0 (plain) or 1 (two-float pairs)");
     if (n == 0) return GPS_OK;
-    const bool persistent = segfp_persistent(h, n, bf16);
+    const bool persistent = segfp_persistent(h, n);
     const gps::TileShape s = persistent && h->size_adaptive ? gps::TileShape{1024, 8} : fp_shape(h, n, true);
     const uint32_t vec_ok = ((ia & (bf16 ? 7u : 15u)) == 0) && ((oa & 15u) == 0);
     hipStream_t st = static_cast<hipStream_t>(stream);

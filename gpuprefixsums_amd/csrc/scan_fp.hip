@@ -732,7 +732,7 @@ __global__ __launch_bounds__(BLOCK) void segscan_fp(SegFpParams p) {
     uint32_t run_f = 0;
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
-        const uint32_t nib = head_nibble(s_heads, run_base + (uint32_t)k * 256u + lane * 4u);
+        const uint32_t nib = run_nibble(s_heads + run_base / 32u + lane / 8u, k, lane);
         nibs[k] = nib;
         ff t = {x[k].x, 0.f};
         t = (nib & 2u) ? ff{x[k].y, 0.f} : ff_add1<COMP>(t, x[k].y);
@@ -874,13 +874,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     while (true) {
         // Scan phase, as segscan_fp: lane-local segmented sums, the segmented
         // wave scan and each lane's carry-in per vector.
+        const uint32_t* lane_words = s_heads[cur] + run_base / 32u + lane / 8u;
         ff lane_carry[VEC];
         uint32_t open = 0;
         ff run = {0.f, 0.f};
         uint32_t run_f = 0;
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
-            const uint32_t nib = head_nibble(s_heads[cur], run_base + (uint32_t)k * 256u + lane * 4u);
+            const uint32_t nib = run_nibble(lane_words, k, lane);
             ff t = {x[k].x, 0.f};
             t = (nib & 2u) ? ff{x[k].y, 0.f} : ff_add1<COMP>(t, x[k].y);
             t = (nib & 4u) ? ff{x[k].z, 0.f} : ff_add1<COMP>(t, x[k].z);
@@ -963,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const ff run_in = pre_f ? pre : ff_add<COMP>({s_ps, s_pc}, pre);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
-            const uint32_t nib = head_nibble(s_heads[cur], run_base + (uint32_t)k * 256u + lane * 4u);
+            const uint32_t nib = run_nibble(lane_words, k, lane);
             ff acc = (open >> k) & 1u ? ff_add<COMP>(run_in, lane_carry[k]) : lane_carry[k];
             f32x4 o;
 #pragma unroll

@@ -58,7 +58,7 @@ __device__ __forceinline__ void seg_scan_run(u32x4 (&d)[VEC], const uint32_t* s_
     carry_flag = 0;
 #pragma unroll
     for (int k = 0; k < VEC; ++k) {
-        const uint32_t nib = head_nibble(s_heads, run_base + (uint32_t)k * 256u + lane * 4u);
+        const uint32_t nib = run_nibble(s_heads + run_base / 32u + lane / 8u, k, lane);
         const u32x4 x = d[k];
         u32x4 v = x;  // lane-local segmented inclusive scan
         v.y = (nib & 2u) ? v.y : v.x + v.y;

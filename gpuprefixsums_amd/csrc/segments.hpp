@@ -73,9 +73,14 @@ __device__ __forceinline__ void mark_heads(uint32_t* s_heads, const uint64_t* of
     mark_heads_by(s_heads, offsets, S, s0, a, b, first_tile, threadIdx.x, BLOCK);
 }
 
-// The 4 head bits of the elements e .. e+3 (e a multiple of 4).
-__device__ __forceinline__ uint32_t head_nibble(const uint32_t* s_heads, uint32_t e) {
-    return (s_heads[e >> 5] >> (e & 31u)) & 0xFu;
+// The 4 head bits of vector k of this lane in a wave's run of the register
+// tile: elements e .. e+3, e = run_base + k * 256 + lane * 4 (run_base a
+// multiple of 256), i.e. bits e % 32 .. +3 of bitmap word e / 32. With
+// `lane_words` = s_heads + run_base / 32 + lane / 8 that word is
+// lane_words[8 k], an immediate LDS offset, so a kernel keeps one address
+// and one shift per lane instead of one word index per vector.
+__device__ __forceinline__ uint32_t run_nibble(const uint32_t* lane_words, int k, uint32_t lane) {
+    return (lane_words[8 * k] >> ((lane & 7u) * 4u)) & 0xFu;
 }
 
 // Inclusive segmented wave scan of (f, v) under

@@ -87,8 +87,10 @@ class Runner:
             e1.synchronize()
             ms = e0.elapsed_time(e1)
             out.append(ms)
-            # long or clearly losing runs need fewer repetitions
-            if ms > 20.0 or (r >= 1 and ms > 4.0 * best_so_far):
+            # long or clearly losing configurations need fewer repetitions;
+            # "clearly losing" is judged on the fastest run so far, so one
+            # transiently slow run cannot end the timing of a fast one
+            if ms > 20.0 or (r >= 1 and min(out) > 4.0 * best_so_far):
                 break
         return statistics.median(out)
 

@@ -79,7 +79,7 @@ def default_config(dtype: str, segments: int, n: int = 1 << 30) -> Config:
         if segments == 1:
             b, v = (256, 4) if n <= 1 << 18 else (1024, 4) if n <= 1 << 22 else (1024, 8)
             return Config("loop_fp", b, v, persistent=n > 1 << 24)
-        if dtype == "f32" and n > 1 << 22:
+        if n > 1 << 22:
             return Config("batched_fp", 1024, 8, persistent=True)
         b, v = (256, 4) if n < 1 << 20 else (1024, 4)
         return Config("batched_fp", b, v)

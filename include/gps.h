@@ -244,8 +244,7 @@ int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch
  * GPS_ERR_UNSUPPORTED). The carry restarts at every segment head at every
  * level (lane, wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. Runs the
  * persistent kernel on a handle with the persistent schedule (a default
- * handle: FP32 input above 2^22 elements, at 1024 x 8), else one workgroup
- * per tile. */
+ * handle: above 2^22 elements, at 1024 x 8), else one workgroup per tile. */
 int gps_segmented_scan_f32(gps_handle* h, const float* d_in, float* d_out, const uint64_t* d_offsets,
                            uint64_t num_segments, uint64_t n, int compensated, void* stream);
 int gps_segmented_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, const uint64_t* d_offsets,

@@ -113,8 +113,8 @@ def test_bf16_logmag_bounds(comp):
 
 @pytest.mark.parametrize("bf16", [False, True])
 def test_default_handle_on_both_sides_of_2p22(bf16):
-    # A default handle switches FP32 input to the persistent 1024 x 8 kernel
-    # above 2^22 elements; BF16 input stays on one workgroup per tile.
+    # A default handle switches to the persistent 1024 x 8 kernel above 2^22
+    # elements.
     for n_target in ((1 << 22) - 1000, (1 << 22) + 40_000):
         off = oracle.gen_lognormal_offsets(13, 700, median=4096.0, sigma=1.0, lo=1, hi=1 << 20)
         off = np.minimum(off, n_target).astype(np.uint64)

@@ -52,7 +52,7 @@ def test_config_names_round_trip_and_candidates():
     assert [default_config("f32", 1, n).name for n in (1 << 18, 1 << 22, (1 << 22) + 1, 1 << 24, (1 << 24) + 1)] == [
         "loop_fp:256x4", "loop_fp:1024x4", "loop_fp:1024x8", "loop_fp:1024x8", "loop_fp:1024x8:pp"]
     assert [default_config("bf16", 9, n).name for n in ((1 << 20) - 1, 1 << 20, 1 << 23)] == [
-        "batched_fp:256x4", "batched_fp:1024x4", "batched_fp:1024x4"]
+        "batched_fp:256x4", "batched_fp:1024x4", "batched_fp:1024x8:pp"]
     assert [default_config("f32", 9, n).name for n in (1 << 22, (1 << 22) + 1)] == [
         "batched_fp:1024x4", "batched_fp:1024x8:pp"]
     assert Config.parse("loop_fp:1024x8:pp") == Config("loop_fp", 1024, 8, persistent=True)
