@@ -3,6 +3,8 @@ limit) for a workload from its size, dtype and segment statistics.
 
     from gpuprefixsums_amd.tuner import Tuner
     cfg = Tuner().choose(workload)          # MLP cost model (trained on the GPU)
+    algo, kw = cfg.handle_args()            # the handle that runs it
+    h = gpuprefixsums_amd.ScanHandle(algo, max_n, **kw)
 
 The cost model is a 3-layer MLP (hidden 64, FP32) trained on timings the
 exhaustive harness (tuner.harness) measured on MI355X; the shipped weights and

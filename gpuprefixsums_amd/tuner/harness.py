@@ -47,12 +47,8 @@ class Runner:
         gps = self.gps
         h = self.handles.get(c)
         if h is None:
-            algo = {"batched": gps.ALGO_CSDL, "loop_csdl": gps.ALGO_CSDL, "loop_rts": gps.ALGO_RTS,
-                    "loop_csdldf": gps.ALGO_CSDLDF, "loop_fp": gps.ALGO_CSDL, "batched_fp": gps.ALGO_CSDL}[c.kind]
-            fp = c.kind in ("loop_fp", "batched_fp")
-            schedule = (gps.SCHEDULE_PERSISTENT if c.persistent else gps.SCHEDULE_TILE) if fp else 0
-            h = gps.ScanHandle(algo, self.max_n, block_threads=c.block, vec_per_thread=c.vec,
-                               spin_limit=c.spin if c.kind == "loop_csdldf" else 0, schedule=schedule)
+            algo, kw = c.handle_args()
+            h = gps.ScanHandle(algo, self.max_n, **kw)
             self.handles[c] = h
         return h
 

@@ -45,6 +45,19 @@ class Config:
         s += f":spin{self.spin}" if self.kind == "loop_csdldf" else ""
         return s + (":pp" if self.persistent else "")
 
+    def handle_args(self):
+        """(algo, keyword arguments) of the gpuprefixsums_amd.ScanHandle that
+        runs this configuration: ScanHandle(algo, max_n, **kwargs)."""
+        import gpuprefixsums_amd as gps
+        algo = {"batched": gps.ALGO_CSDL, "loop_csdl": gps.ALGO_CSDL, "loop_rts": gps.ALGO_RTS,
+                "loop_csdldf": gps.ALGO_CSDLDF, "loop_fp": gps.ALGO_CSDL, "batched_fp": gps.ALGO_CSDL}[self.kind]
+        kw = dict(block_threads=self.block, vec_per_thread=self.vec)
+        if self.kind == "loop_csdldf":
+            kw["spin_limit"] = self.spin
+        if self.kind in ("loop_fp", "batched_fp"):
+            kw["schedule"] = gps.SCHEDULE_PERSISTENT if self.persistent else gps.SCHEDULE_TILE
+        return algo, kw
+
     @staticmethod
     def parse(name: str) -> "Config":
         parts = name.split(":")

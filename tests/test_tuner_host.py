@@ -125,3 +125,13 @@ def test_gated_choice_policy():
     assert gated_choice(cs, pred, d, 0.05) == cs[1]   # 11% predicted gain > 5%
     assert gated_choice(cs, pred, d, 0.2) == d        # < 20%: keep the default
     assert gated_choice(cs[1:], pred[1:], d, 0.2) == cs[1]  # default not a candidate: plain argmin
+
+
+def test_config_handle_args():
+    import gpuprefixsums_amd as gps
+    assert Config("loop_fp", 1024, 8, persistent=True).handle_args() == (
+        gps.ALGO_CSDL, dict(block_threads=1024, vec_per_thread=8, schedule=gps.SCHEDULE_PERSISTENT))
+    assert Config("batched_fp", 256, 4).handle_args()[1]["schedule"] == gps.SCHEDULE_TILE
+    assert Config("loop_csdldf", 1024, 4, 32).handle_args() == (
+        gps.ALGO_CSDLDF, dict(block_threads=1024, vec_per_thread=4, spin_limit=32))
+    assert Config("loop_rts", 512, 2).handle_args() == (gps.ALGO_RTS, dict(block_threads=512, vec_per_thread=2))
