@@ -219,6 +219,9 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
         return fail(GPS_ERR_INVALID, "unsupported tile shape block=%u vec=%u (block 256/512/1024, vec 2/4/8)",
                     shape.block, shape.vec);
     if (stall_period == 1) return fail(GPS_ERR_INVALID, "stall_period must be 0 or >= 2");
+    // A waiter polls up to 8 x spin_limit times; the bound keeps that product
+    // in 32 bits and below the device watchdog's 2^20 polls.
+    if (spin_limit > 65536) return fail(GPS_ERR_INVALID, "spin_limit must be at most 65536");
     if (stall_period && algo != GPS_ALGO_CSDLDF)
         return fail(GPS_ERR_INVALID, "stall emulation needs the decoupled-fallback algorithm");
     // One status slot (kSlotWords words) per kMinTileElems elements: every

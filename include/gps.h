@@ -53,7 +53,7 @@ typedef struct gps_tuning {
     uint32_t vec_per_thread; /* uint4 vectors per thread: 2, 4 or 8 (items = 4x) */
     uint32_t spin_limit;     /* CSDLDF: polls without progress before a fallback (default 4), for the
                                 blocking tile's nearest successor; a waiter k tiles farther waits
-                                (1 + min(k, 7)) x spin_limit polls */
+                                (1 + min(k, 7)) x spin_limit polls; at most 65536 */
     uint32_t stall_period;   /* CSDLDF: stall tiles with index % period == 0 (0 = no emulation, >= 2) */
     uint32_t schedule;       /* CSDL, CSDLDF: GPS_SCHEDULE_* (0 = default) */
 } gps_tuning;

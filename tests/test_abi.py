@@ -52,6 +52,12 @@ def test_argument_errors_before_any_device_call(gps_lib):
     assert gps_lib.gps_create(0, 1024, ctypes.byref(stall_on_csdl), ctypes.byref(h)) == GPS_ERR_INVALID
     period_one = GpsTuning(0, 0, 0, 1)
     assert gps_lib.gps_create(2, 1024, ctypes.byref(period_one), ctypes.byref(h)) == GPS_ERR_INVALID
+    huge_spin = GpsTuning(0, 0, 65537, 8)
+    assert gps_lib.gps_create(2, 1024, ctypes.byref(huge_spin), ctypes.byref(h)) == GPS_ERR_INVALID
+    assert b"spin_limit" in gps_lib.gps_last_error()
+    persistent, control = GpsTuning(0, 0, 0, 0, 2), GpsTuning(0, 0, 0, 0, 3)
+    assert gps_lib.gps_create(1, 1024, ctypes.byref(persistent), ctypes.byref(h)) == GPS_ERR_INVALID  # RTS
+    assert gps_lib.gps_create(2, 1024, ctypes.byref(control), ctypes.byref(h)) == GPS_ERR_INVALID  # CSDLDF
     assert gps_lib.gps_inclusive_scan_u32(None, None, None, 0, None) == GPS_ERR_INVALID
     assert gps_lib.gps_destroy(None) == GPS_ERR_INVALID
     assert gps_lib.gps_version().startswith(b"gpuprefixsums_amd")
