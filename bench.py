@@ -52,6 +52,29 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def host_cpu() -> dict:
+    """The host CPU as lscpu reports it, the logical CPUs the OS shows, and the
+    CPUs this process may run on (its affinity mask)."""
+    info = {"model": cpu_model(), "logical_cpus": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "logical_cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, ValueError):
+        pass
+    try:
+        info["cpus_allowed"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["cpus_allowed"] = os.cpu_count()
+    return info
+
+
 def gbps(n: int, ms: float, bytes_per_elem: float = BYTES_PER_ELEM) -> float:
     return n * bytes_per_elem / (ms * 1e-3) / 1e9
 
@@ -127,11 +150,12 @@ class Legs:
         """rocPRIM's device scan (the reference's CUB comparator on ROCm) on the
         same input, separate buffers, inclusive, checked before it is reported."""
         gps, torch = self.gps, self.torch
-        tb = gps.vendor_scan_temp_bytes(self.n, True)
+        from gpuprefixsums_amd import vendor
+        tb = vendor.vendor_scan_temp_bytes(self.n, True)
         temp = torch.empty(max(tb, 4), dtype=torch.uint8, device="cuda")
         for _ in range(3):
-            gps.vendor_scan_u32(self.d_in, self.d_out, self.n, temp, tb, True)
-        ms = gps.vendor_time_scan_u32(self.d_in, self.d_out, self.n, temp, tb, True, iters=20)
+            vendor.vendor_scan_u32(self.d_in, self.d_out, self.n, temp, tb, True)
+        ms = vendor.vendor_time_scan_u32(self.d_in, self.d_out, self.n, temp, tb, True, iters=20)
         torch.cuda.synchronize()
         errors = gps.validate_scan_u32(self.d_in, self.d_out, self.n, True)
         mean = sum(ms) / len(ms)
@@ -217,23 +241,29 @@ class Legs:
             if h.device_error():
                 raise RuntimeError("device error in the segmented FP leg")
         y64 = oracle.segmented_scan_fp_f64(xh, off)
-        rel = lambda y: np.abs(y.cpu().numpy().astype(np.float64) - y64) / np.maximum(np.abs(y64), 1.0)  # noqa: E731
-        ra, rb = rel(a), rel(b)
+        s64 = oracle.segmented_scan_fp_f64(xh, off, absolute=True)
+        err = lambda y: np.abs(y.cpu().numpy().astype(np.float64) - y64)  # noqa: E731
+        ea, eb = err(a), err(b)
+        ra, rb = ea / np.maximum(np.abs(y64), 1.0), eb / np.maximum(np.abs(y64), 1.0)
+        wa, wb = ea / np.maximum(s64, np.finfo(np.float64).tiny), eb / np.maximum(s64, np.finfo(np.float64).tiny)
         nbytes = 8 * n + 8 * (S + 1)
         out["segmented_fp"] = {
             "segments": S, "n_total": n, "batched_ms": round(batched_ms, 4),
             "per_segment_launches_ms": round(loop_ms, 3), "speedup": round(loop_ms / batched_ms, 1),
             "batched_gbps": round(nbytes / batched_ms / 1e6, 1),
             "max_rel_err": {"batched": float(ra.max()), "per_segment": float(rb.max())},
-            "median_rel_err": {"batched": float(np.median(ra)), "per_segment": float(np.median(rb))}}
+            "median_rel_err": {"batched": float(np.median(ra)), "per_segment": float(np.median(rb))},
+            "max_normwise_err": {"batched": float(wa.max()), "per_segment": float(wb.max())},
+            "error_measures": "rel_i = |y_i - y64_i| / max(|y64_i|, 1); normwise_i = |y_i - y64_i| / S_i with "
+                              "S_i = sum of |x_j| from the segment head through i"}
 
     def fp(self, out):
         """n = 2^27 inclusive FP scans: FP32 ~ N(0,1) (seed 4) and BF16 with
         magnitudes 10^U(-4,4) and random sign (seed 5); carries as two-float
-        TwoSum pairs ("compensated"), as FP64 ("f64carry") or as plain FP32
-        ("plain"); GB/s (8 B/elem FP32, 6 B/elem BF16 in + FP32 out) and max /
-        median relative error against the FP64 oracle,
-        rel_i = |y_i - y64_i| / max(|y64_i|, 1)."""
+        TwoSum pairs ("compensated") or as plain FP32 ("plain"); GB/s (8 B/elem
+        FP32, 6 B/elem BF16 in + FP32 out) and two error measures against the
+        FP64 oracle: max / median rel_i = |y_i - y64_i| / max(|y64_i|, 1), and
+        max normwise_i = |y_i - y64_i| / S_i with S_i = sum_{j<=i} |x_j|."""
         gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
         n = 1 << 27
         st = torch.cuda.current_stream()
@@ -243,16 +273,17 @@ class Legs:
             for kind in ("f32", "bf16"):
                 if kind == "f32":
                     x = oracle.gen_f32_normal(4, n)
-                    y64 = oracle.scan_f32_f64(x)
+                    y64, s64 = oracle.scan_f32_f64(x), oracle.abs_scan_f32_f64(x)
                     din = torch.from_numpy(x).cuda()
                     fn, bpe = h.inclusive_f32, 8
                 else:
                     x = oracle.gen_bf16_logmag(5, n)
-                    y64 = oracle.scan_bf16_f64(x)
+                    y64, s64 = oracle.scan_bf16_f64(x), oracle.abs_scan_bf16_f64(x)
                     din = torch.from_numpy(x.view(np.int16)).cuda()
                     fn, bpe = h.inclusive_bf16_f32, 6
                 den = np.maximum(np.abs(y64), 1.0)
-                for comp in (gps.FP_PAIR, gps.FP_F64, gps.FP_PLAIN):
+                sden = np.maximum(s64, np.finfo(np.float64).tiny)
+                for comp in (gps.FP_PAIR, gps.FP_PLAIN):
                     for _ in range(3):
                         fn(din, dout, n, comp)
                     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -262,12 +293,14 @@ class Legs:
                     e1.record(st)
                     torch.cuda.synchronize()
                     ms = e0.elapsed_time(e1) / 10
-                    rel = np.abs(dout.cpu().numpy().astype(np.float64) - y64) / den
-                    mode = {gps.FP_PAIR: "compensated", gps.FP_F64: "f64carry", gps.FP_PLAIN: "plain"}[comp]
+                    err = np.abs(dout.cpu().numpy().astype(np.float64) - y64)
+                    rel = err / den
+                    mode = {gps.FP_PAIR: "compensated", gps.FP_PLAIN: "plain"}[comp]
                     res[f"{kind}_{mode}"] = {
                         "gbps": round(n * bpe / ms / 1e6, 1), "max_rel_err": float(rel.max()),
-                        "median_rel_err": float(np.median(rel))}
-                del x, y64, din, den
+                        "median_rel_err": float(np.median(rel)), "max_normwise_err": float((err / sden).max())}
+                    del err, rel
+                del x, y64, s64, din, den, sden
             if h.device_error():
                 raise RuntimeError("device error in fp leg")
         out["fp_n2p27"] = res
@@ -482,18 +515,27 @@ def main() -> int:
             oracle.lib().orc_scan_u32(x, out, n, 1)
             runs.append(time.perf_counter() - t)
         s = statistics.median(runs)
+        host = host_cpu()
         cpu = {"value": round(n * BYTES_PER_ELEM / s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"full workload n=2^{args.log2n} inclusive, median of 3 runs "
-                         f"({s * 1e3:.0f} ms each) on {cpu_model()}"}
-        threads = min(16, os.cpu_count() or 1)
+               "sample": f"full workload n=2^{args.log2n} inclusive, serial scan on one core, median of 3 runs "
+                         f"({s * 1e3:.0f} ms each) on {host['model']}",
+               "host": host}
+        # The blocked two-pass scan on every CPU this process is granted: the
+        # affinity mask, further capped by OMP_NUM_THREADS where the launcher
+        # sets the per-GPU CPU share there.
+        threads = host["cpus_allowed"] or 1
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
         runs = []
         for _ in range(3):
             t = time.perf_counter()
             oracle.lib().orc_scan_u32_mt(x, out, n, 1, threads)
             runs.append(time.perf_counter() - t)
         s = statistics.median(runs)
-        extras["cpu_all_cores"] = {"value": round(n * BYTES_PER_ELEM / s / 1e9, 3), "unit": "GB/s",
-                                   "cores": threads, "kind": "port"}
+        extras["cpu_parallel"] = {"value": round(n * BYTES_PER_ELEM / s / 1e9, 3), "unit": "GB/s",
+                                  "threads": threads, "kind": "port",
+                                  "sample": "blocked two-pass scan (tile sums, serial spine, rescan) on the CPUs "
+                                            "granted to this process (affinity mask, capped by OMP_NUM_THREADS)"}
         log(f"cpu baseline: {json.dumps(cpu)}")
 
     line = {

@@ -24,9 +24,8 @@ __all__ = [
     "ALGO_CSDL", "ALGO_RTS", "ALGO_CSDLDF", "GpsError", "ScanHandle",
     "ChainedScanDecoupledLookbackDispatcher", "ReduceThenScanDispatcher",
     "EmulatedDeadlockingDispatcher", "fill_u32", "fill_random_u32", "validate_scan_u32",
-    "validate_ones_u32", "load", "vendor_scan_temp_bytes", "vendor_scan_u32", "vendor_time_scan_u32",
-    "CubDispatcher", "SCHEDULE_DEFAULT", "SCHEDULE_TILE", "SCHEDULE_PERSISTENT", "SCHEDULE_CONTROL",
-    "FP_PLAIN", "FP_PAIR", "FP_F64", "WAVE_INCLUSIVE_SCAN", "WAVE_SUM", "WAVE_EXCLUSIVE_SCAN",
+    "validate_ones_u32", "load", "SCHEDULE_DEFAULT", "SCHEDULE_TILE", "SCHEDULE_PERSISTENT", "SCHEDULE_CONTROL",
+    "FP_PLAIN", "FP_PAIR", "WAVE_INCLUSIVE_SCAN", "WAVE_SUM", "WAVE_EXCLUSIVE_SCAN",
     "wave_primitive_u32", "SURVEY_NETWORKS", "survey_segment", "survey_scan_u32",
 ]
 
@@ -41,8 +40,15 @@ def _ptr(x) -> int:
     raise TypeError(f"expected a device pointer or tensor, got {type(x)!r}")
 
 
-def _stream(s) -> Optional[int]:
+def _stream(s, *bufs) -> Optional[int]:
+    """The stream to launch on: `s` if given, else torch's current stream of
+    the first device tensor among `bufs` (so tensor callers get stream-ordered
+    launches), else the null stream."""
     if s is None:
+        for b in bufs:
+            if getattr(b, "is_cuda", False):
+                import torch
+                return int(torch.cuda.current_stream(b.device).cuda_stream)
         return None
     if isinstance(s, int):
         return s
@@ -55,6 +61,18 @@ def _numel(x) -> int:
     if hasattr(x, "numel"):
         return int(x.numel())
     raise TypeError("n must be given for raw pointers")
+
+
+def _need(what: str, nbytes: int, *bufs) -> None:
+    """Raises ValueError when a sized buffer (anything with numel() and
+    element_size(), e.g. a torch tensor) holds fewer than nbytes bytes, so an
+    explicit n can never make a kernel run past a buffer. Raw addresses are
+    the caller's responsibility."""
+    for b in bufs:
+        if b is not None and hasattr(b, "numel") and hasattr(b, "element_size"):
+            have = int(b.numel()) * int(b.element_size())
+            if have < nbytes:
+                raise ValueError(f"{what}: a buffer holds {have} bytes but the call needs {nbytes}")
 
 
 class ScanHandle:
@@ -105,79 +123,98 @@ class ScanHandle:
 
     def inclusive(self, d_in, d_out, n: Optional[int] = None, stream=None) -> None:
         n = _numel(d_in) if n is None else n
-        check(load().gps_inclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
+        _need("inclusive", 4 * n, d_in, d_out)
+        check(load().gps_inclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream, d_in, d_out)))
 
     def exclusive(self, d_in, d_out, n: Optional[int] = None, stream=None) -> None:
         n = _numel(d_in) if n is None else n
-        check(load().gps_exclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
+        _need("exclusive", 4 * n, d_in, d_out)
+        check(load().gps_exclusive_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream, d_in, d_out)))
 
     def scan(self, d_in, d_out, n: Optional[int] = None, inclusive: bool = True, stream=None) -> None:
         n = _numel(d_in) if n is None else n
-        check(load().gps_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive), _stream(stream)))
+        _need("scan", 4 * n, d_in, d_out)
+        check(load().gps_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive),
+                                  _stream(stream, d_in, d_out)))
 
     def scan_carry(self, d_in, d_out, n: int, inclusive: bool = True, d_carry_in=None, d_carry_out=None,
                    stream=None) -> None:
         """Scan continued from a device carry word (d_carry_in, may be None);
         d_carry_out (may be None, may equal d_carry_in) receives the total plus
         the carry."""
+        _need("scan_carry", 4 * n, d_in, d_out)
+        _need("scan_carry", 4, d_carry_in, d_carry_out)
         check(load().gps_scan_u32_carry(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive),
-                                        _ptr(d_carry_in), _ptr(d_carry_out), _stream(stream)))
+                                        _ptr(d_carry_in), _ptr(d_carry_out), _stream(stream, d_in, d_out)))
 
     def time(self, d_in, d_out, n: int, inclusive: bool = True, iters: int = 10, stream=None) -> list:
         ms = (ctypes.c_float * iters)()
+        _need("time", 4 * n, d_in, d_out)
         check(load().gps_time_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), n, int(inclusive),
-                                       _stream(stream), iters, ms))
+                                       _stream(stream, d_in, d_out), iters, ms))
         return list(ms)
 
     def scan_u32x4(self, d_in, d_out, n: int, inclusive: bool = True, stream=None) -> None:
         """Member-wise scan of n 4-member uint32 structs (uint4 elements)."""
         fn = load().gps_inclusive_scan_u32x4 if inclusive else load().gps_exclusive_scan_u32x4
-        check(fn(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream)))
+        _need("scan_u32x4", 16 * n, d_in, d_out)
+        check(fn(self.handle, _ptr(d_in), _ptr(d_out), n, _stream(stream, d_in, d_out)))
 
     def estimate_occupancy(self, d_in, d_out, n: int, struct_scan: bool = False, stream=None) -> tuple:
         """Runs the persistent inclusive scan with one workgroup per tile and
         returns (workgroups that took a tile, resident workgroups of the normal
         persistent launch)."""
         measured, resident = ctypes.c_uint32(), ctypes.c_uint32()
+        _need("estimate_occupancy", (16 if struct_scan else 4) * n, d_in, d_out)
         check(load().gps_estimate_occupancy(self.handle, _ptr(d_in), _ptr(d_out), n, int(struct_scan),
-                                            _stream(stream), ctypes.byref(measured), ctypes.byref(resident)))
+                                            _stream(stream, d_in, d_out), ctypes.byref(measured),
+                                            ctypes.byref(resident)))
         return measured.value, resident.value
 
     def inclusive_f32(self, d_in, d_out, n: Optional[int] = None, compensated=True, stream=None) -> None:
         """FP32 -> FP32 inclusive scan. compensated: False/0 plain FP32 carries,
-        True/1 two-float TwoSum pairs, FP_F64 (2) FP64 carries."""
+        True/1 two-float TwoSum pairs."""
         n = _numel(d_in) if n is None else n
+        _need("inclusive_f32", 4 * n, d_in, d_out)
         check(load().gps_inclusive_scan_f32(self.handle, _ptr(d_in), _ptr(d_out), n, int(compensated),
-                                            _stream(stream)))
+                                            _stream(stream, d_in, d_out)))
 
     def inclusive_bf16_f32(self, d_in, d_out, n: Optional[int] = None, compensated=True,
                            stream=None) -> None:
         """BF16 (uint16 bits) -> FP32 inclusive scan (compensated as inclusive_f32)."""
         n = _numel(d_in) if n is None else n
+        _need("inclusive_bf16_f32", 2 * n, d_in)
+        _need("inclusive_bf16_f32", 4 * n, d_out)
         check(load().gps_inclusive_scan_bf16_f32(self.handle, _ptr(d_in), _ptr(d_out), n, int(compensated),
-                                                 _stream(stream)))
+                                                 _stream(stream, d_in, d_out)))
 
     def segmented(self, d_in, d_out, d_offsets, num_segments: int, n: int, inclusive: bool = False,
                   stream=None) -> None:
         """Batched segmented scan in one launch (d_offsets: device uint64[num_segments + 1])."""
+        _need("segmented", 4 * n, d_in, d_out)
+        _need("segmented", 8 * (num_segments + 1), d_offsets)
         check(load().gps_segmented_scan_u32(self.handle, _ptr(d_in), _ptr(d_out), _ptr(d_offsets), num_segments,
-                                            n, int(inclusive), _stream(stream)))
+                                            n, int(inclusive), _stream(stream, d_in, d_out)))
 
     def segmented_per_segment(self, d_in, d_out, h_offsets, inclusive: bool = False, stream=None) -> None:
         """Baseline: one plain scan launch per segment (h_offsets: host uint64 sequence)."""
         import numpy as np
         off = np.ascontiguousarray(h_offsets, dtype=np.uint64)
         ptr = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        _need("segmented_per_segment", 4 * int(off[-1]) if off.size else 0, d_in, d_out)
         check(load().gps_segmented_scan_u32_per_segment(self.handle, _ptr(d_in), _ptr(d_out), ptr, off.size - 1,
-                                                        int(inclusive), _stream(stream)))
+                                                        int(inclusive), _stream(stream, d_in, d_out)))
 
     def segmented_fp(self, d_in, d_out, d_offsets, num_segments: int, n: int, bf16: bool = False,
                      compensated: bool = True, stream=None) -> None:
         """Batched segmented FP inclusive scan in one launch (FP32 or BF16-bits
         input, FP32 output; d_offsets: device uint64[num_segments + 1])."""
         fn = load().gps_segmented_scan_bf16_f32 if bf16 else load().gps_segmented_scan_f32
+        _need("segmented_fp", (2 if bf16 else 4) * n, d_in)
+        _need("segmented_fp", 4 * n, d_out)
+        _need("segmented_fp", 8 * (num_segments + 1), d_offsets)
         check(fn(self.handle, _ptr(d_in), _ptr(d_out), _ptr(d_offsets), num_segments, n, int(compensated),
-                 _stream(stream)))
+                 _stream(stream, d_in, d_out)))
 
     def fp_per_segment(self, d_in, d_out, h_offsets, bf16: bool = False, compensated: bool = True,
                        stream=None) -> None:
@@ -185,12 +222,17 @@ class ScanHandle:
         import numpy as np
         off = np.ascontiguousarray(h_offsets, dtype=np.uint64)
         ptr = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        total = int(off[-1]) if off.size else 0
+        _need("fp_per_segment", (2 if bf16 else 4) * total, d_in)
+        _need("fp_per_segment", 4 * total, d_out)
         check(load().gps_inclusive_scan_fp_per_segment(self.handle, _ptr(d_in), _ptr(d_out), ptr, off.size - 1,
-                                                       int(bf16), int(compensated), _stream(stream)))
+                                                       int(bf16), int(compensated), _stream(stream, d_in, d_out)))
 
     def segmented_graph(self, d_in, d_out, d_offsets, num_segments: int, n: int,
                         inclusive: bool = False) -> "Graph":
         g = ctypes.c_void_p()
+        _need("segmented_graph", 4 * n, d_in, d_out)
+        _need("segmented_graph", 8 * (num_segments + 1), d_offsets)
         check(load().gps_segmented_graph_create(self.handle, _ptr(d_in), _ptr(d_out), _ptr(d_offsets),
                                                 num_segments, n, int(inclusive), ctypes.byref(g)))
         return Graph(g)
@@ -269,7 +311,7 @@ def validate_ones_u32(d_scan, n: int, inclusive: bool = True, stream=None) -> in
 WAVE_INCLUSIVE_SCAN, WAVE_SUM, WAVE_EXCLUSIVE_SCAN = 0, 1, 2
 
 # FP scan carries (gps.h GPS_FP_*).
-FP_PLAIN, FP_PAIR, FP_F64 = 0, 1, 2
+FP_PLAIN, FP_PAIR = 0, 1
 
 
 def wave_primitive_u32(d_in, d_out, n: int, op: int, stream=None) -> None:
@@ -297,30 +339,6 @@ def survey_segment(alg: int) -> int:
 def survey_scan_u32(alg: int, d_in, d_out, n: int, inclusive: bool = True, stream=None) -> None:
     """Scans each segment of n elements independently with one survey network."""
     check(load().gps_survey_scan_u32(alg, _ptr(d_in), _ptr(d_out), n, int(inclusive), _stream(stream)))
-
-
-# ---- vendor comparator (rocPRIM), CubDispatcher.cuh:32-76 ------------------
-
-def vendor_scan_temp_bytes(n: int, inclusive: bool = True) -> int:
-    b = ctypes.c_uint64()
-    check(load().gps_vendor_scan_temp_bytes(n, int(inclusive), ctypes.byref(b)))
-    return b.value
-
-
-def vendor_scan_u32(d_in, d_out, n: int, d_temp, temp_bytes: int, inclusive: bool = True, stream=None) -> None:
-    """rocPRIM's device scan of n uint32 (d_in == d_out allowed); d_temp holds
-    at least vendor_scan_temp_bytes(n, inclusive) bytes."""
-    check(load().gps_vendor_scan_u32(_ptr(d_in), _ptr(d_out), n, int(inclusive), _ptr(d_temp), temp_bytes,
-                                     _stream(stream)))
-
-
-def vendor_time_scan_u32(d_in, d_out, n: int, d_temp, temp_bytes: int, inclusive: bool = True, iters: int = 10,
-                         stream=None) -> list:
-    """Per-launch milliseconds of `iters` back-to-back rocPRIM scans."""
-    ms = (ctypes.c_float * iters)()
-    check(load().gps_vendor_time_scan_u32(_ptr(d_in), _ptr(d_out), n, int(inclusive), _ptr(d_temp), temp_bytes,
-                                          _stream(stream), iters, ms))
-    return list(ms)
 
 
 # ---- reference dispatcher classes (self-tests run in the C++ classes) -----
@@ -354,17 +372,3 @@ class ReduceThenScanDispatcher(_Dispatcher):
 
 class EmulatedDeadlockingDispatcher(_Dispatcher):
     _algo = ALGO_CSDLDF
-
-
-class CubDispatcher:
-    """The reference's vendor comparator (CubDispatcher.cuh:21-76) on rocPRIM:
-    BatchTimingCubChainedScan times the vendor inclusive scan in place over
-    all-ones input, first iteration discarded, and returns keys/s."""
-
-    def __init__(self, max_size: int):
-        self.max_size = max_size
-
-    def batch_timing_cub_chained_scan(self, size: int, batch_count: int) -> float:
-        keys = ctypes.c_double()
-        check(load().gps_vendor_batch_timing(self.max_size, size, batch_count, ctypes.byref(keys)))
-        return keys.value

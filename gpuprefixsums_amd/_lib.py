@@ -52,7 +52,8 @@ class GpsError(RuntimeError):
 
 
 # name -> (restype, argtypes). Every function declared in include/gps.h must
-# be listed here; tests/test_abi.py checks both directions.
+# be listed here; tests/test_abi.py checks both directions. (The vendor
+# comparator's entry points, include/gps_vendor.h, are bound in vendor.py.)
 SIGNATURES = {
     "gps_version": (ctypes.c_char_p, []),
     "gps_last_error": (ctypes.c_char_p, []),
@@ -98,10 +99,6 @@ SIGNATURES = {
     "gps_validate_scan_u32": (_int, [_vp, _vp, _u64, _int, _vp, ctypes.POINTER(_u64)]),
     "gps_dispatcher_test_all": (_int, [_int, _u32, _int, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "gps_wave_primitive_u32": (_int, [_vp, _vp, _u64, _int, _vp]),
-    "gps_vendor_scan_temp_bytes": (_int, [_u64, _int, ctypes.POINTER(_u64)]),
-    "gps_vendor_scan_u32": (_int, [_vp, _vp, _u64, _int, _vp, _u64, _vp]),
-    "gps_vendor_time_scan_u32": (_int, [_vp, _vp, _u64, _int, _vp, _u64, _vp, _u32, _fp]),
-    "gps_vendor_batch_timing": (_int, [_u32, _u32, _u32, ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None

@@ -1,8 +1,11 @@
 """Build recipe for libgps.so (gfx950) and the CPU oracle.
 
-Compiles every HIP/C++ source under csrc/ with hipcc for gfx950 into
-gpuprefixsums_amd/libgps.so, the reference-style test program gpuprefixsums_amd/gps_main,
-and the oracle (oracle/liboracle.so, test infrastructure) with make.
+Compiles the HIP/C++ sources under csrc/ with hipcc for gfx950 into
+gpuprefixsums_amd/libgps.so (the product library), the vendor comparator
+gpuprefixsums_amd/libgps_vendor.so (rocPRIM's scan, for tests and benchmarks
+only; it links libgps.so), the reference-style test program
+gpuprefixsums_amd/gps_main, and the oracle (oracle/liboracle.so, test
+infrastructure) with make.
 
     python -m gpuprefixsums_amd.build          # build everything
 """
@@ -21,6 +24,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libgps.so"
+VENDOR_LIB = PKG / "libgps_vendor.so"
 MAIN = PKG / "gps_main"
 ARCH = "gfx950"
 
@@ -31,12 +35,17 @@ LIB_SOURCES = [
     "segmented_u32.hip",
     "scan_fp.hip",
     "mlp.hip",
-    "vendor_rocprim.hip",
     "scan_u32x4.hip",
     "survey.hip",
     "capi.cpp",
     "capi_ext.cpp",
     "dispatchers.cpp",
+]
+
+# The comparator library: rocPRIM's device scan and the CubDispatcher class.
+VENDOR_SOURCES = [
+    "vendor_rocprim.hip",
+    "vendor_dispatcher.cpp",
 ]
 
 COMMON_FLAGS = [
@@ -77,17 +86,23 @@ def _compile(src: str) -> Path:
     return obj
 
 
-def build_lib(jobs: int = 8) -> Path:
-    BUILD.mkdir(exist_ok=True)
-    sources = [s for s in LIB_SOURCES if (CSRC / s).exists()]
-    with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(sources)))) as ex:
-        objs = list(ex.map(_compile, sources))
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+def _link(objs, out: Path, extra=()) -> Path:
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), *extra]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+    return out
+
+
+def build_lib(jobs: int = 8) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    sources = [s for s in LIB_SOURCES + VENDOR_SOURCES if (CSRC / s).exists()]
+    with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(sources)))) as ex:
+        objs = dict(zip(sources, ex.map(_compile, sources)))
+    _link([objs[s] for s in LIB_SOURCES], LIB)
+    _link([objs[s] for s in VENDOR_SOURCES], VENDOR_LIB, [f"-L{PKG}", "-lgps", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 
@@ -95,7 +110,7 @@ def build_main() -> Path:
     src = CSRC / "gps_main.cpp"
     if not src.exists():
         return MAIN
-    cmd = [hipcc(), *COMMON_FLAGS, "-x", "hip", str(src), "-o", str(MAIN), f"-L{PKG}", "-lgps",
+    cmd = [hipcc(), *COMMON_FLAGS, "-x", "hip", str(src), "-o", str(MAIN), f"-L{PKG}", "-lgps", "-lgps_vendor",
            f"-Wl,-rpath,$ORIGIN"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

@@ -136,8 +136,8 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
         if (carry_out) {
             if (carry_in)
                 HIP_TRY(hipMemcpyAsync(carry_out, carry_in, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-            else
-                HIP_TRY(hipMemsetAsync(carry_out, 0, sizeof(uint32_t), s));
+            else  // a fill kernel, not a memset: the call stays capturable into a graph
+                HIP_TRY(gps::launch_fill_u32(carry_out, 1, 0u, s));
         }
         return GPS_OK;
     }
@@ -313,9 +313,17 @@ int gps_reset_counters(gps_handle* h) {
 
 int gps_device_error(gps_handle* h, uint32_t* bits) {
     if (!h || !bits) return fail(GPS_ERR_INVALID, "null argument");
+    if (int rc_ = gps::check_device(h)) return rc_;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(bits, h->err, sizeof(uint32_t), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemset(h->err, 0, sizeof(uint32_t)));
+    if (*bits) {
+        // A launch that raised an error may have left tiles unpublished or its
+        // ticket word short of the count that re-arms it: start the next
+        // launch from cleared status words and a fresh ticket word.
+        HIP_TRY(gps::launch_reset_status(h->status, (uint64_t)h->max_tiles * gps::kSlotWords, h->ticket, nullptr));
+        h->launches = 0;
+    }
     HIP_TRY(hipDeviceSynchronize());
     return GPS_OK;
 }

@@ -90,8 +90,8 @@ int scan_fp(gps_handle* h, const void* in, float* out, uint64_t n, bool bf16, in
     if (n > h->max_n)
         return fail(GPS_ERR_TOO_LARGE, "n=%llu exceeds handle max_n=%llu", (unsigned long long)n,
                     (unsigned long long)h->max_n);
-    if (compensated < gps::kFpPlain || compensated > gps::kFpF64)
-        return fail(GPS_ERR_INVALID, "compensated must be 0 (plain), 1 (two-float pairs) or 2 (FP64 carries)");
+    if (compensated < gps::kFpPlain || compensated > gps::kFpPair)
+        return fail(GPS_ERR_INVALID, "compensated must be 0 (plain) or 1 (two-float pairs)");
     const uintptr_t ia = reinterpret_cast<uintptr_t>(in), oa = reinterpret_cast<uintptr_t>(out);
     if ((ia & (bf16 ? 1u : 3u)) || (oa & 3u)) return fail(GPS_ERR_INVALID, "misaligned buffer");
     if (n == 0) return GPS_OK;
@@ -127,8 +127,6 @@ int segscan_fp(gps_handle* h, const void* in, float* out, const uint64_t* offset
     const uintptr_t ia = reinterpret_cast<uintptr_t>(in), oa = reinterpret_cast<uintptr_t>(out);
     if ((ia & (bf16 ? 1u : 3u)) || (oa & 3u)) return fail(GPS_ERR_INVALID, "misaligned buffer");
     if (reinterpret_cast<uintptr_t>(offsets) & 7u) return fail(GPS_ERR_INVALID, "offsets must be 8-byte aligned");
-    if (compensated == gps::kFpF64)
-        return fail(GPS_ERR_UNSUPPORTED, "FP64 carries are offered for the unsegmented FP scan only");
     if (compensated < gps::kFpPlain || compensated > gps::kFpPair)
         return fail(GPS_ERR_INVALID, "compensated must be 0 (plain) or 1 (two-float pairs)");
     if (n == 0) return GPS_OK;

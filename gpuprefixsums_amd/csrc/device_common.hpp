@@ -134,6 +134,22 @@ __device__ __forceinline__ Ticket take_ticket(uint64_t* ticket_word, uint32_t to
     return t;
 }
 
+// Called by one thread of a persistent workgroup that stops early (a watchdog
+// fired) while holding ticket `held`: takes tickets until it holds one past
+// the last tile, as a normal exit would, so the launch still takes exactly
+// `total_tickets` and the holder of the last one re-arms the word. Tiles
+// taken here are never scanned; the launch has already raised an error.
+__device__ __forceinline__ void drain_tickets(uint64_t* ticket_word, uint64_t held, uint32_t num_tiles,
+                                              uint32_t total_tickets) {
+    uint64_t t = held;
+    while (true) {
+        const uint32_t c = (uint32_t)t;
+        if (c == total_tickets - 1u) atomic_store_agent(ticket_word, (uint64_t)next_epoch((uint32_t)(t >> 32)) << 32);
+        if (c >= num_tiles) return;
+        t = __hip_atomic_fetch_add(ticket_word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Device-side watchdog: a spin that exceeds this many empty polls records an
 // error (read back by the host) and gives up instead of spinning forever.
 // Every 1024 polls a spinner also reads the error word, so once one

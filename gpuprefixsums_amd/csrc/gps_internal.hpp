@@ -102,7 +102,7 @@ hipError_t launch_segscan_u32(TileShape s, bool inclusive, bool persistent, cons
 // FP32 / BF16-input inclusive scans with FP32 output (2 status words per tile).
 bool fp_shape_supported(TileShape s);
 // Carries of the FP scans (values match GPS_FP_* in gps.h).
-enum FpCarry : int { kFpPlain = 0, kFpPair = 1, kFpF64 = 2 };
+enum FpCarry : int { kFpPlain = 0, kFpPair = 1 };
 // persistent: resident workgroups loop over ticketed tiles, loading tile i+1
 // while tile i's lookback resolves (otherwise, and always for FP64 carries,
 // one workgroup per tile).

@@ -10,6 +10,7 @@
 #include <cstdlib>
 
 #include "../../include/gps_dispatchers.hpp"
+#include "../../include/gps_vendor.hpp"
 
 int main(int argc, char** argv) {
     const uint32_t log2n = argc > 1 ? (uint32_t)atoi(argv[1]) : 28u;

@@ -13,12 +13,6 @@
 // its error is about one FP32 rounding of the exact prefix, independent of n.
 // The plain mode is the same kernel with single floats (c == 0).
 //
-// FP64-carry mode (scan_fp_f64) carries the same partials as one double
-// instead of a pair. The FP32 inputs convert exactly; a double add is one
-// half-rate VALU instruction, against the eight instructions of a pair add,
-// and the 53-bit significand keeps the accumulated error far below one FP32
-// rounding. Each output is the running double rounded once to FP32.
-//
 // This file is compiled with -ffp-contract=off: a fused multiply-add would
 // break the TwoSum error terms.
 #include "gps_internal.hpp"
@@ -280,164 +274,6 @@ __global__ __launch_bounds__(BLOCK) void scan_fp(FpParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64-carry scan: the pair kernel's schedule with every partial above the
-// element level held as one double. Status: word 0 of a tile's slot holds the
-// low half of the double, word 1 the high half, both with the flag and epoch;
-// readers accept a tile when the two flags agree, as for the pair words.
-// ---------------------------------------------------------------------------
-
-// DPP move of a double: both 32-bit halves with the same controls, so a lane
-// that a step does not feed receives +0.0.
-#define GPS_DPP_D(x, CTRL, RM, BC)                                                                        \
-    __builtin_bit_cast(double,                                                                            \
-                       ((uint64_t)(uint32_t)__builtin_amdgcn_update_dpp(                                  \
-                            0, (int)(uint32_t)(__builtin_bit_cast(uint64_t, (x)) >> 32), CTRL, RM, 0xf, BC) \
-                        << 32) |                                                                          \
-                           (uint64_t)(uint32_t)__builtin_amdgcn_update_dpp(                               \
-                               0, (int)(uint32_t)__builtin_bit_cast(uint64_t, (x)), CTRL, RM, 0xf, BC))
-
-__device__ __forceinline__ double wave_incl_scan_d(double x) {
-    x += GPS_DPP_D(x, 0x111, 0xf, true);   // row_shr:1
-    x += GPS_DPP_D(x, 0x112, 0xf, true);   // row_shr:2
-    x += GPS_DPP_D(x, 0x114, 0xf, true);   // row_shr:4
-    x += GPS_DPP_D(x, 0x118, 0xf, true);   // row_shr:8
-    x += GPS_DPP_D(x, 0x142, 0xa, false);  // row_bcast:15
-    x += GPS_DPP_D(x, 0x143, 0xc, false);  // row_bcast:31
-    return x;
-}
-
-__device__ __forceinline__ double readlane_d(double x, int lane) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-__device__ __forceinline__ void publish_d(uint64_t* status, uint32_t tile, double v, uint64_t flag, uint32_t epoch) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    atomic_store_agent(status + kSlotWords * tile, make_status((uint32_t)b, flag, epoch));
-    atomic_store_agent(status + kSlotWords * tile + 1, make_status((uint32_t)(b >> 32), flag, epoch));
-}
-
-// Window lookback over the double words (one full wave): the exclusive
-// prefix of `tile`.
-__device__ __forceinline__ double lookback_d(const uint64_t* status, uint32_t tile, uint32_t epoch, uint32_t* err) {
-    const uint32_t lane = lane_id();
-    double prefix = 0.0;
-    int64_t top = (int64_t)tile - 1;
-    uint32_t empty_polls = 0;
-    while (true) {
-        // Branch-free window read (lanes below tile 0 reload tile 0 and take
-        // the sentinel), so no divergent region precedes the ballots.
-        const int64_t idx = top - (int64_t)lane;
-        const int64_t at = idx >= 0 ? idx : 0;
-        const uint64_t l0 = atomic_load_agent(status + kSlotWords * at);
-        const uint64_t l1 = atomic_load_agent(status + kSlotWords * at + 1);
-        const uint64_t sentinel = make_status(0u, kFlagIncl, epoch);
-        const uint64_t w0 = idx >= 0 ? l0 : sentinel, w1 = idx >= 0 ? l1 : sentinel;
-        const uint32_t f0 = status_flag(w0, epoch), f1 = status_flag(w1, epoch);
-        const uint32_t f = f0 == f1 ? f0 : 0u;
-        const uint64_t ready = ballot(f != 0u);
-        const uint64_t incl = ballot(f == (uint32_t)kFlagIncl);
-        const uint32_t run = (~ready) ? (uint32_t)__builtin_ctzll(~ready) : 64u;
-        const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-        const uint32_t take = first_incl < run ? first_incl + 1u : run;
-        if (take) {
-            const double v = __builtin_bit_cast(double, ((uint64_t)(uint32_t)w1 << 32) | (uint32_t)w0);
-            const double s = wave_incl_scan_d(lane < take ? v : 0.0);
-            prefix += readlane_d(s, 63);
-            if (first_incl < run) return prefix;
-            top -= run;
-            empty_polls = 0;
-        } else {
-            if (watchdog_expired(empty_polls, err)) {
-                if (lane == 0) raise_device_error(err, kErrLookbackWatchdog);
-                return prefix;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-}
-
-template <int BLOCK, int VEC, bool BF16>
-__global__ __launch_bounds__(BLOCK) void scan_fp_f64(FpParams p) {
-    using C = TileCfg<BLOCK, VEC>;
-    __shared__ double s_w[C::kWaves];
-    __shared__ uint32_t s_ticket[2];
-    __shared__ double s_p;
-    const uint32_t wave = wave_index();
-    const uint32_t lane = lane_id();
-    if (threadIdx.x == 0) {
-        const Ticket t = take_ticket(p.ticket, p.num_tiles);
-        s_ticket[0] = t.tile;
-        s_ticket[1] = t.epoch;
-    }
-    __syncthreads();
-    const uint32_t tile = s_ticket[0], epoch = s_ticket[1];
-    const uint64_t base = thread_base<BLOCK, VEC>(tile);
-    const bool vec_path = p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n);
-    f32x4 x[VEC];
-    load_fp<BF16, VEC>(x, p.in, base, p.n, vec_path);
-
-    // Per vector: lane total, wave exclusive scan -> lane carry.
-    double lane_carry[VEC];
-    double run = 0.0;
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-        const double t = (((double)x[k].x + (double)x[k].y) + (double)x[k].z) + (double)x[k].w;
-        const double s = wave_incl_scan_d(t);
-        lane_carry[k] = run + GPS_DPP_D(s, 0x138, 0xf, true);  // wave_shr:1, lane 0 gets 0
-        run += readlane_d(s, 63);
-    }
-    if (lane == 0) s_w[wave] = run;
-    __syncthreads();
-    double wave_pre, tile_tot;
-    {
-        const double v = s_w[lane < C::kWaves ? lane : 0u];
-        const double sc = wave_incl_scan_d(lane < C::kWaves ? v : 0.0);
-        wave_pre = readlane_d(GPS_DPP_D(sc, 0x138, 0xf, true), (int)wave);
-        tile_tot = readlane_d(sc, C::kWaves - 1);
-    }
-    if (wave == 0) {
-        double prefix = 0.0;
-        if (tile == 0) {
-            if (lane == 0) publish_d(p.status, 0, tile_tot, kFlagIncl, epoch);
-        } else {
-            if (lane == 0) publish_d(p.status, tile, tile_tot, kFlagAgg, epoch);
-            prefix = lookback_d(p.status, tile, epoch, p.err);
-            if (lane == 0) publish_d(p.status, tile, prefix + tile_tot, kFlagIncl, epoch);
-        }
-        if (lane == 0) s_p = prefix;
-    }
-    __syncthreads();
-    const double base_carry = s_p + wave_pre;
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-        double acc = base_carry + lane_carry[k];
-        f32x4 y;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            acc += (double)x[k][c];
-            y[c] = (float)acc;
-        }
-        x[k] = y;
-    }
-    if (vec_path) {
-#pragma unroll
-        for (int k = 0; k < VEC; ++k)
-            __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
-    } else {
-#pragma unroll
-        for (int k = 0; k < VEC; ++k)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
-                if (i < p.n) p.out[i] = x[k][c];
-            }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Persistent, software-pipelined FP scan (the uint32 csdl_pp_u32 schedule):
 // resident workgroups loop over ticketed tiles; with tile i scanned and its
 // aggregate published, a workgroup takes its next ticket and issues tile
@@ -447,9 +283,7 @@ __global__ __launch_bounds__(BLOCK) void scan_fp_f64(FpParams p) {
 //
 // 2^27 FP32 / BF16 elements at 1024 x 8: pairs 4998 / 4433 GB/s against
 // 4787 / 4009 for one workgroup per tile, plain 5567 / 5285 against 5383 /
-// 4921; at 2^23 one workgroup per tile is faster. FP64 carries have no
-// persistent form: with two register tiles they spill at 1024 x 8, and at
-// 1024 x 4 they ran 4722 against 5067 GB/s for scan_fp_f64 at 1024 x 8.
+// 4921; at 2^23 one workgroup per tile is faster.
 // ---------------------------------------------------------------------------
 template <bool COMP>
 struct PairCarry {
@@ -1040,14 +874,10 @@ hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, co
     p.err = err;
 #define X(B, V)                                                                                                 \
     if (s.block == B && s.vec == V) {                                                                           \
-        if (persistent && carry != kFpF64)                                                                      \
+        if (persistent)                                                                                         \
             return bf16 ? launch_fp_pp_carry<B, V, true>(carry, p, stream)                                      \
                         : launch_fp_pp_carry<B, V, false>(carry, p, stream);                                    \
-        if (carry == kFpF64 && bf16)                                                                            \
-            hipLaunchKernelGGL((scan_fp_f64<B, V, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);         \
-        else if (carry == kFpF64)                                                                               \
-            hipLaunchKernelGGL((scan_fp_f64<B, V, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);        \
-        else if (bf16 && compensated)                                                                           \
+        if (bf16 && compensated)                                                                                \
             hipLaunchKernelGGL((scan_fp<B, V, true, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);       \
         else if (bf16)                                                                                          \
             hipLaunchKernelGGL((scan_fp<B, V, true, false>), dim3(p.num_tiles), dim3(B), 0, stream, p);      \

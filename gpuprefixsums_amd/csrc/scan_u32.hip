@@ -291,7 +291,11 @@ csdl_cw_u32(ScanParams p) {
             if (lane == 0) old = __hip_atomic_fetch_add(p.ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bool ok = true;
             if (lane < (uint32_t)DW) ok = lds_wait_ge(&s_arrived[lane], i + 1u, p.err);
-            if (ballot(!ok)) return;
+            if (ballot(!ok)) {
+                // Give up, but keep the ticket count exact so the word re-arms.
+                if (lane == 0) drain_tickets(p.ticket, old, p.num_tiles, total_tickets);
+                return;
+            }
             const uint32_t total = wave_sum(lane < (uint32_t)DW ? s_tot[i & 1u][lane] : 0u);
             if (lane == 0) {
                 atomic_store_agent(p.status + kSlotWords * tile, make_status(total, tile ? kFlagAgg : kFlagIncl, epoch));

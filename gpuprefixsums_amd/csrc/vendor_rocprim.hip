@@ -1,6 +1,7 @@
 // Vendor comparator: rocPRIM's device-wide scan (rocprim::inclusive_scan /
 // exclusive_scan, rocPRIM 4.2 in ROCm 7.2) behind the same kind of C entry
-// points as the library's own scans.
+// points as the library's own scans (include/gps_vendor.h). Built into the
+// comparator library libgps_vendor.so, never into libgps.so.
 //
 // Replaces the reference's CUB timing comparator
 // (GPUPrefixSumsCUDA/CubDispatcher.cuh:32-76), which times
@@ -15,6 +16,7 @@
 #include <rocprim/device/device_scan.hpp>
 #include <vector>
 
+#include "../../include/gps_vendor.h"
 #include "handle.hpp"
 
 using gps::fail;

@@ -195,17 +195,13 @@ int gps_survey_scan_u32(int alg, const uint32_t* d_in, uint32_t* d_out, uint64_t
  * the inter-tile lookback chain) is carried:
  *   GPS_FP_PLAIN (0): one FP32 float, the plain chained scan;
  *   GPS_FP_PAIR  (1): a two-float pair combined with TwoSum, each output
- *                     rounded once;
- *   GPS_FP_F64   (2): one FP64 value, each output rounded once (same
- *                     accuracy as the pair, fewer instructions).
- * Other values return GPS_ERR_INVALID. One kernel launch; needs a
- * GPS_ALGO_CSDL handle. Plain and pair carries run the persistent kernel on a
- * handle with the persistent schedule (a default handle: above 2^24
- * elements), else one workgroup per tile; FP64 carries always run one
- * workgroup per tile. */
+ *                     rounded once.
+ * Other values return GPS_ERR_INVALID. No FP64 arithmetic is used. One
+ * kernel launch; needs a GPS_ALGO_CSDL handle. Runs the persistent kernel on
+ * a handle with the persistent schedule (a default handle: above 2^24
+ * elements), else one workgroup per tile. */
 #define GPS_FP_PLAIN 0
 #define GPS_FP_PAIR 1
-#define GPS_FP_F64 2
 int gps_inclusive_scan_f32(gps_handle* h, const float* d_in, float* d_out, uint64_t n, int compensated,
                            void* stream);
 int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_out, uint64_t n, int compensated,
@@ -240,8 +236,7 @@ int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch
  * segments: segment s covers [d_offsets[s], d_offsets[s+1]) (device array of
  * num_segments + 1 entries, d_offsets[0] = 0, d_offsets[S] = n; empty
  * segments allowed). Same input/output types as the plain FP scans;
- * compensated is GPS_FP_PLAIN or GPS_FP_PAIR (GPS_FP_F64 returns
- * GPS_ERR_UNSUPPORTED). The carry restarts at every segment head at every
+ * compensated is GPS_FP_PLAIN or GPS_FP_PAIR. The carry restarts at every segment head at every
  * level (lane, wave, tile, lookback). Needs a GPS_ALGO_CSDL handle. Runs the
  * persistent kernel on a handle with the persistent schedule (a default
  * handle: above 2^22 elements, at 1024 x 8), else one workgroup per tile. */
@@ -263,7 +258,10 @@ int gps_get_counters(gps_handle* h, uint64_t out[4]);
 int gps_reset_counters(gps_handle* h);
 /* Device error bits raised by watchdogs since the last call (and clears
  * them): 1 = a lookback exceeded its poll budget, 2 = an emulated stall did.
- * Synchronises the device. */
+ * Synchronises the device. When any bit was set it also re-arms the handle's
+ * status words and ticket word (as gps_reset does), because the failed launch
+ * may have left tiles unpublished or tickets untaken; the next launch then
+ * starts clean. */
 int gps_device_error(gps_handle* h, uint32_t* bits);
 /* Re-arms the status words (epoch wrap after 2^30-1 launches; the direct entry
  * points do this themselves, graph replays must call it). */
@@ -301,29 +299,12 @@ int gps_validate_scan_u32(const uint32_t* d_in, const uint32_t* d_scan, uint64_t
 enum { GPS_WAVE_INCLUSIVE_SCAN = 0, GPS_WAVE_SUM = 1, GPS_WAVE_EXCLUSIVE_SCAN = 2 };
 int gps_wave_primitive_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int op, void* stream);
 
-/* ---- vendor comparator: rocPRIM's device scan ----------------------------
- * Replaces the reference's CUB comparator (CubDispatcher.cuh:32-76:
- * cub::DeviceScan::InclusiveSum with a two-phase temporary-storage query).
- * On ROCm the vendor scan is rocprim::inclusive_scan / exclusive_scan.
- * Caller buffers (d_in == d_out allowed), caller temporary storage of at
- * least gps_vendor_scan_temp_bytes(n, inclusive) bytes, no handle. */
-int gps_vendor_scan_temp_bytes(uint64_t n, int inclusive, uint64_t* bytes);
-int gps_vendor_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* d_temp,
-                        uint64_t temp_bytes, void* stream);
-/* `iters` back-to-back vendor scans, per-launch hipEvent times into ms[iters]
- * (the timing loop of CubDispatcher.cuh:52-68). */
-int gps_vendor_time_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* d_temp,
-                             uint64_t temp_bytes, void* stream, uint32_t iters, float* ms);
-
 /* ---- reference dispatcher self-tests (C++ classes in gps_dispatchers.hpp) - */
 
 /* Runs <Dispatcher>::TestAllInclusive / TestAllExclusive for algo with a
  * dispatcher of max_size elements; writes tests passed and tests run.
  * (ChainedScanDecoupledLookbackDispatcher.cuh:38-114) */
 int gps_dispatcher_test_all(int algo, uint32_t max_size, int inclusive, uint32_t* passed, uint32_t* total);
-/* Runs CubDispatcher(max_size).BatchTimingCubChainedScan(size, batch_count)
- * on the rocPRIM comparator and writes its keys/s (CubDispatcher.cuh:32-76). */
-int gps_vendor_batch_timing(uint32_t max_size, uint32_t size, uint32_t batch_count, double* keys_per_sec);
 
 #ifdef __cplusplus
 }

@@ -102,22 +102,4 @@ class EmulatedDeadlockingDispatcher : public DispatcherBase {
     uint32_t m_stallPeriod, m_spinLimit;
 };
 
-// The vendor-library comparator (CubDispatcher.cuh:21-76). On ROCm the vendor
-// scan is rocPRIM's (gps_vendor_scan_u32); the class keeps the reference's
-// name and method so callers switch unchanged. Like the reference it times
-// the in-place inclusive scan of all-ones input, re-filled before every
-// iteration, first iteration discarded; it returns keys/s.
-class CubDispatcher {
-   public:
-    explicit CubDispatcher(uint32_t size);
-    ~CubDispatcher();
-    CubDispatcher(const CubDispatcher&) = delete;
-    CubDispatcher& operator=(const CubDispatcher&) = delete;
-    double BatchTimingCubChainedScan(uint32_t size, uint32_t batchCount);
-
-   private:
-    const uint32_t k_maxSize;
-    uint32_t* m_scan = nullptr;
-};
-
 }  // namespace gps

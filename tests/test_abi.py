@@ -12,13 +12,26 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 
 
+# The comparator library's header; every other include/*.h belongs to libgps.so.
+VENDOR_HEADER = "gps_vendor.h"
+
+
+def declared_in(h: Path) -> set:
+    text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+    return {m.group(1) for m in re.finditer(r"\b(gps_[a-z0-9_]+)\s*\(", text)}
+
+
 def declared_functions() -> set:
+    """Entry points of the product library libgps.so."""
     names = set()
     for h in (ROOT / "include").glob("*.h"):
-        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
-        for m in re.finditer(r"\b(gps_[a-z0-9_]+)\s*\(", text):
-            names.add(m.group(1))
+        if h.name != VENDOR_HEADER:
+            names |= declared_in(h)
     return names
+
+
+def vendor_functions() -> set:
+    return declared_in(ROOT / "include" / VENDOR_HEADER)
 
 
 def test_header_declares_entry_points():
@@ -33,9 +46,31 @@ def test_library_exports_every_declared_symbol(gps_lib):
 
 
 def test_python_binding_covers_header():
-    from gpuprefixsums_amd import _lib
+    from gpuprefixsums_amd import _lib, vendor
     assert declared_functions() == set(_lib.SIGNATURES), (
         declared_functions() ^ set(_lib.SIGNATURES))
+    assert vendor_functions() == set(vendor.SIGNATURES), vendor_functions() ^ set(vendor.SIGNATURES)
+
+
+def test_comparator_library_exports_its_header(gps_lib):
+    from gpuprefixsums_amd import build, vendor
+    if not vendor.VENDOR_LIB_PATH.exists():
+        build.build_lib()
+    v = vendor.load()
+    missing = [n for n in sorted(vendor_functions()) if not hasattr(v, n)]
+    assert not missing, f"libgps_vendor.so lacks {missing}"
+
+
+def test_product_library_carries_no_vendor_scan(gps_lib):
+    import shutil
+    import subprocess
+    from gpuprefixsums_amd import _lib
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm is not installed")
+    syms = subprocess.run([nm, "-D", str(_lib.LIB_PATH)], capture_output=True, text=True, check=True).stdout
+    assert "rocprim" not in syms.lower()
+    assert not any(hasattr(gps_lib, n) for n in vendor_functions())
 
 
 def test_argument_errors_before_any_device_call(gps_lib):

@@ -5,9 +5,9 @@ Error measures (stated with every result):
           inputs have unit median magnitude, so prefixes near a zero crossing
           are judged in absolute terms)
   S_i   = sum_{j <= i} |x_j|                 (for normwise bounds)
-Modes: plain FP32 carries (0), two-float TwoSum pairs (1), FP64 carries (2).
+Modes: plain FP32 carries (0), two-float TwoSum pairs (1).
 Bounds asserted:
-  compensated (pairs or FP64):
+  compensated (pairs):
                  |y_i - y64_i| <= 2^-23 |y64_i| + 2^-36 S_i
                  (one FP32 rounding of the exact prefix, plus the u^2-level
                  error of the pair arithmetic)
@@ -26,8 +26,8 @@ torch = pytest.importorskip("torch")
 
 import gpuprefixsums_amd as gps  # noqa: E402
 
-MODES = [gps.FP_PAIR, gps.FP_PLAIN, gps.FP_F64]
-MODE_IDS = ["pair", "plain", "f64"]
+MODES = [gps.FP_PAIR, gps.FP_PLAIN]
+MODE_IDS = ["pair", "plain"]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -77,8 +77,7 @@ def test_integer_valued_inputs_are_exact(bf16, comp):
 @pytest.mark.parametrize("comp", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("n", [1, 3, 4095, 4097, 65537, (1 << 22) + 7])
 def test_f32_normal_bounds(n, comp, shape, schedule):
-    # One workgroup per tile, or the persistent pipelined kernel (pair and
-    # plain carries; FP64 carries run one workgroup per tile on both).
+    # One workgroup per tile, or the persistent pipelined kernel.
     x = oracle.gen_f32_normal(4, n)
     y64 = oracle.scan_f32_f64(x)
     S = oracle.abs_scan_f32_f64(x)
@@ -108,14 +107,11 @@ def test_compensation_beats_plain_fp32():
     y64 = oracle.scan_f32_f64(x)
     with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
         yc = run(h, x, False, gps.FP_PAIR)
-        yd = run(h, x, False, gps.FP_F64)
         yp = run(h, x, False, gps.FP_PLAIN)
     ec = np.abs(yc - y64) / np.maximum(np.abs(y64), 1.0)
-    ed = np.abs(yd - y64) / np.maximum(np.abs(y64), 1.0)
     ep = np.abs(yp - y64) / np.maximum(np.abs(y64), 1.0)
     assert np.median(ec) < np.median(ep) and ec.max() < ep.max()
-    assert np.median(ed) < np.median(ep) and ed.max() < ep.max()
-    assert ec.max() <= 2.0 ** -23 and ed.max() <= 2.0 ** -23
+    assert ec.max() <= 2.0 ** -23
 
 
 @pytest.mark.parametrize("bf16", [False, True])
@@ -136,19 +132,19 @@ def test_config4_n2p27(bf16):
             del y
         assert h.device_error() == 0
     print(f"n=2^27 {'bf16' if bf16 else 'f32'}: pair max/median rel {report[gps.FP_PAIR]}, "
-          f"FP64 carries {report[gps.FP_F64]}, plain {report[gps.FP_PLAIN]}")
-    for comp in (gps.FP_PAIR, gps.FP_F64):
-        assert report[comp][0] <= 2.0 ** -23 and report[comp][0] < report[gps.FP_PLAIN][0]
+          f"plain {report[gps.FP_PLAIN]}")
+    assert report[gps.FP_PAIR][0] <= 2.0 ** -23 and report[gps.FP_PAIR][0] < report[gps.FP_PLAIN][0]
 
 
 def test_carry_mode_errors():
     d = torch.zeros(1024, dtype=torch.float32, device="cuda")
     off = torch.tensor([0, 1024], dtype=torch.int64, device="cuda")
     with gps.ScanHandle(gps.ALGO_CSDL, 1024) as h:
-        with pytest.raises(gps.GpsError, match="compensated"):
-            h.inclusive_f32(d, d, 1024, 3)
-        with pytest.raises(gps.GpsError, match="FP64"):
-            h.segmented_fp(d, d, off, 1, 1024, compensated=gps.FP_F64)
-        # the per-segment loop runs the unsegmented kernel, FP64 carries included
-        h.fp_per_segment(d, d, [0, 1024], compensated=gps.FP_F64)
+        for bad in (2, 3, -1):  # only plain (0) and pairs (1) exist; no FP64 mode
+            with pytest.raises(gps.GpsError, match="compensated"):
+                h.inclusive_f32(d, d, 1024, bad)
+            with pytest.raises(gps.GpsError, match="compensated"):
+                h.segmented_fp(d, d, off, 1, 1024, compensated=bad)
+            with pytest.raises(gps.GpsError, match="compensated"):
+                h.fp_per_segment(d, d, [0, 1024], compensated=bad)
         torch.cuda.synchronize()

@@ -274,3 +274,87 @@ def test_persistent_schedule_exercises_the_fallback_branches():
             acc = (acc + agg[t]) & MASK32
     for branch in ("fallback", "cas_win", "cas_lost_agg", "fallback_tile0", "stall_released"):
         assert stats[branch] > 0, (branch, dict(stats))
+
+
+# ---------------------------------------------------------------------------
+# Ticket-word accounting (device_common.hpp: take_ticket, drain_tickets).
+# The word holds (epoch << 32) | tickets_taken; a persistent launch takes
+# num_tiles + grid tickets (each workgroup ends on one past the last tile) and
+# the holder of the last re-arms the word to (next epoch, 0). A workgroup that
+# gives up early (a watchdog) drains tickets until it holds one past the last
+# tile, so the count still reaches the total.
+# ---------------------------------------------------------------------------
+def next_epoch(e):
+    n = e + 1
+    if (n & EPOCH_MASK) == 0:
+        n += 1
+    return n
+
+
+class TicketWord:
+    def __init__(self, epoch):
+        self.v = epoch << 32
+
+    def fetch_add(self):
+        old = self.v
+        self.v += 1
+        return old
+
+    def rearm_if_last(self, old, total):
+        if (old & MASK32) == total - 1:
+            self.v = next_epoch(old >> 32) << 32
+
+
+def persistent_tickets(word, num_tiles, total, give_up_after, drain):
+    """Generator for one workgroup: takes tickets until one past the last tile;
+    with give_up_after = k it stops after holding its k-th ticket, draining
+    (drain_tickets) or simply returning."""
+    taken = 0
+    while True:
+        old = word.fetch_add()
+        word.rearm_if_last(old, total)
+        taken += 1
+        yield
+        if (old & MASK32) >= num_tiles:
+            return
+        if give_up_after is not None and taken >= give_up_after:
+            if drain:
+                t = old
+                while (t & MASK32) < num_tiles:
+                    t = word.fetch_add()
+                    word.rearm_if_last(t, total)
+                    yield
+            return
+
+
+def run_ticket_launch(word, num_tiles, grid, give_up, rng, drain=True):
+    total = num_tiles + grid
+    progs = [persistent_tickets(word, num_tiles, total, give_up.get(g), drain) for g in range(grid)]
+    while progs:
+        p = rng.choice(progs)
+        try:
+            next(p)
+        except StopIteration:
+            progs.remove(p)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_ticket_word_rearms_after_early_exits(seed):
+    rng = random.Random(9000 + seed)
+    epoch = rng.choice([1, 7, EPOCH_MASK - 1, EPOCH_MASK])
+    word = TicketWord(epoch)
+    for launch in range(5):
+        num_tiles, grid = rng.choice([1, 3, 40, 200]), rng.choice([1, 2, 8, 33])
+        give_up = {g: rng.randint(1, 4) for g in range(grid) if rng.random() < 0.3}
+        run_ticket_launch(word, num_tiles, grid, give_up, rng)
+        epoch = next_epoch(epoch)
+        assert word.v == epoch << 32, (seed, launch, hex(word.v))
+
+
+def test_early_exit_without_draining_leaves_the_word_armed_wrong():
+    # The failure the drain prevents: a workgroup that returns while holding a
+    # tile leaves the count short, so the word is never re-armed.
+    rng = random.Random(1)
+    word = TicketWord(3)
+    run_ticket_launch(word, 40, 4, {0: 1}, rng, drain=False)
+    assert word.v >> 32 == 3 and (word.v & MASK32) < 44
