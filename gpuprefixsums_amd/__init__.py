@@ -17,14 +17,15 @@ import ctypes
 from typing import Optional, Sequence
 
 from . import _lib
-from ._lib import (ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, SCHEDULE_CONTROL, SCHEDULE_DEFAULT, SCHEDULE_PERSISTENT, SCHEDULE_TILE,
-                   GpsError, GpsTuning, check, load)
+from ._lib import (ALGO_CSDL, ALGO_CSDLDF, ALGO_RTS, SCHEDULE_CONTROL, SCHEDULE_DEFAULT, SCHEDULE_PERSISTENT,
+                   SCHEDULE_SPLIT, SCHEDULE_TILE, GpsError, GpsTuning, check, load)
 
 __all__ = [
     "ALGO_CSDL", "ALGO_RTS", "ALGO_CSDLDF", "GpsError", "ScanHandle",
     "ChainedScanDecoupledLookbackDispatcher", "ReduceThenScanDispatcher",
     "EmulatedDeadlockingDispatcher", "fill_u32", "fill_random_u32", "validate_scan_u32",
     "validate_ones_u32", "load", "SCHEDULE_DEFAULT", "SCHEDULE_TILE", "SCHEDULE_PERSISTENT", "SCHEDULE_CONTROL",
+    "SCHEDULE_SPLIT",
     "FP_PLAIN", "FP_PAIR", "WAVE_INCLUSIVE_SCAN", "WAVE_SUM", "WAVE_EXCLUSIVE_SCAN",
     "wave_primitive_u32", "SURVEY_NETWORKS", "survey_segment", "survey_scan_u32",
 ]

@@ -43,6 +43,7 @@ SCHEDULE_DEFAULT = 0
 SCHEDULE_TILE = 1
 SCHEDULE_PERSISTENT = 2
 SCHEDULE_CONTROL = 3
+SCHEDULE_SPLIT = 4
 
 
 class GpsError(RuntimeError):

@@ -170,9 +170,14 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
     p.carry_in = carry_in;
     p.carry_out = carry_out;
     if (h->algo == GPS_ALGO_CSDLDF) {
-        if (schedule == GPS_SCHEDULE_PERSISTENT)
+        if (schedule == GPS_SCHEDULE_PERSISTENT) {
             HIP_TRY(gps::launch_csdldf_pp_u32(shape, inclusive, p, s));
-        else
+        } else if (schedule == GPS_SCHEDULE_SPLIT) {
+            // Both launches share one epoch: the second is counted here too,
+            // so no host re-arm can fall between them.
+            HIP_TRY(gps::before_epoch_launch(h, s));
+            HIP_TRY(gps::launch_csdldf_split_u32(shape, inclusive, p, s));
+        } else
             HIP_TRY(gps::launch_csdldf_u32(shape, inclusive, p, s));
     } else if (schedule == GPS_SCHEDULE_PERSISTENT) {
         HIP_TRY(gps::launch_csdl_pp_u32(shape, inclusive, p, s));
@@ -207,7 +212,9 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
         stall_period = tuning->stall_period;
         schedule = tuning->schedule;
     }
-    if (schedule > GPS_SCHEDULE_CONTROL) return fail(GPS_ERR_INVALID, "unknown schedule %u", schedule);
+    if (schedule > GPS_SCHEDULE_SPLIT) return fail(GPS_ERR_INVALID, "unknown schedule %u", schedule);
+    if (schedule == GPS_SCHEDULE_SPLIT && (algo != GPS_ALGO_CSDLDF || stall_period < 2))
+        return fail(GPS_ERR_INVALID, "the split schedule needs the decoupled-fallback algorithm and stall_period >= 2");
     if (schedule == GPS_SCHEDULE_PERSISTENT && algo == GPS_ALGO_RTS)
         return fail(GPS_ERR_INVALID, "the persistent schedule is offered for CSDL and CSDLDF");
     if (schedule == GPS_SCHEDULE_CONTROL && algo != GPS_ALGO_CSDL)

@@ -134,6 +134,26 @@ __device__ __forceinline__ Ticket take_ticket(uint64_t* ticket_word, uint32_t to
     return t;
 }
 
+// Epoch before `e` (both masked), the inverse of next_epoch: the second
+// launch of a split scan runs under the first launch's epoch, after that
+// launch's last ticket has re-armed the word.
+__device__ __forceinline__ uint32_t prev_epoch(uint32_t e) {
+    uint32_t p = (e - 1u) & kEpochMask;
+    if (p == 0u) p = kEpochMask;
+    return p;
+}
+
+// Split schedule (two launches; stall_period P >= 2, tiles >= 2): launch 1
+// takes every tile with index % P != 0 in order, launch 2 the deferred tiles
+// 0, P, 2P, ... Ticket q of a launch maps to its q-th tile.
+__host__ __device__ __forceinline__ uint32_t split_tile_count(uint32_t num_tiles, uint32_t period, uint32_t pass) {
+    const uint32_t deferred = (num_tiles + period - 1u) / period;
+    return pass == 2u ? deferred : num_tiles - deferred;
+}
+__device__ __forceinline__ uint32_t split_tile(uint32_t q, uint32_t period, uint32_t pass) {
+    return pass == 2u ? q * period : q + q / (period - 1u) + 1u;
+}
+
 // Called by one thread of a persistent workgroup that stops early (a watchdog
 // fired) while holding ticket `held`: takes tickets until it holds one past
 // the last tile, as a normal exit would, so the launch still takes exactly

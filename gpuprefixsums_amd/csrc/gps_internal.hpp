@@ -41,6 +41,10 @@ struct ScanParams {
     // two may be the same word (chunked scans chain through it).
     const uint32_t* carry_in;
     uint32_t* carry_out;
+    // CSDLDF split schedule (two launches, the D3D12 deadlock emulation):
+    // 0 = not split, 1 = first launch (tiles with index % stall_period != 0),
+    // 2 = second launch (the deferred tiles, under the first launch's epoch)
+    uint32_t split_pass;
 };
 
 // Workgroups of KERNEL (BLOCK threads) resident on the current device at
@@ -84,6 +88,9 @@ uint32_t pp_resident_workgroups(TileShape s, bool inclusive);
 uint32_t cw_tile_elems(TileShape s);
 hipError_t launch_csdl_cw_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
+// Split schedule: the first launch (every tile except index % stall_period
+// == 0) and then the second (the deferred tiles), back to back on `stream`.
+hipError_t launch_csdldf_split_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 // Persistent decoupled-fallback scan (grid = resident workgroups).
 hipError_t launch_csdldf_pp_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream);
 hipError_t launch_rts_u32(TileShape s, bool inclusive, const uint32_t* in, uint32_t* out, uint64_t n,

@@ -32,9 +32,15 @@ struct TileFront {
     __device__ __forceinline__ void run(const ScanParams& p, uint32_t* s_wave, uint32_t* s_ticket) {
         const uint32_t wave = wave_index();
         if (threadIdx.x == 0) {
-            const Ticket t = take_ticket(p.ticket, p.num_tiles);
-            s_ticket[0] = t.tile;
-            s_ticket[1] = t.epoch;
+            if (p.split_pass) {
+                const Ticket t = take_ticket(p.ticket, split_tile_count(p.num_tiles, p.stall_period, p.split_pass));
+                s_ticket[0] = split_tile(t.tile, p.stall_period, p.split_pass);
+                s_ticket[1] = p.split_pass == 2u ? prev_epoch(t.epoch) : t.epoch;
+            } else {
+                const Ticket t = take_ticket(p.ticket, p.num_tiles);
+                s_ticket[0] = t.tile;
+                s_ticket[1] = t.epoch;
+            }
         }
         __syncthreads();
         tile = s_ticket[0];
@@ -411,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
 
     if (threadIdx.x == 0) {
         unsigned long long stall_timeouts = 0;
-        if (p.stall_period && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
+        if (p.stall_period && !p.split_pass && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
             uint32_t polls = 0;
             while (status_flag(atomic_load_agent(p.status + kSlotWords * tile), epoch) == 0u) {
                 if (watchdog_expired(polls, p.err)) {
@@ -620,7 +626,7 @@ csdldf_pp_u32(ScanParams p) {
         uint32_t wave_prefix, tile_total;
         combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
         if (threadIdx.x == 0) {
-            if (p.stall_period && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
+            if (p.stall_period && !p.split_pass && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
                 uint32_t polls = 0;
                 while (status_flag(atomic_load_agent(p.status + kSlotWords * tile), epoch) == 0u) {
                     if (watchdog_expired(polls, p.err)) {
@@ -1070,18 +1076,37 @@ hipError_t launch_csdl_cw_u32(TileShape s, bool inclusive, const ScanParams& p, 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+static hipError_t launch_csdldf_grid(TileShape s, bool inclusive, const ScanParams& p, uint32_t grid,
+                                     hipStream_t stream) {
 #define X(B, V)                                                                            \
     if (s.block == B && s.vec == V) {                                                      \
         if (inclusive)                                                                     \
-            hipLaunchKernelGGL((csdldf_u32<B, V, true>), dim3(p.num_tiles), dim3(B), 0, stream, p);  \
+            hipLaunchKernelGGL((csdldf_u32<B, V, true>), dim3(grid), dim3(B), 0, stream, p);  \
         else                                                                               \
-            hipLaunchKernelGGL((csdldf_u32<B, V, false>), dim3(p.num_tiles), dim3(B), 0, stream, p); \
+            hipLaunchKernelGGL((csdldf_u32<B, V, false>), dim3(grid), dim3(B), 0, stream, p); \
         return hipGetLastError();                                                          \
     }
     GPS_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_csdldf_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+    return launch_csdldf_grid(s, inclusive, p, p.num_tiles, stream);
+}
+
+hipError_t launch_csdldf_split_u32(TileShape s, bool inclusive, const ScanParams& p, hipStream_t stream) {
+    ScanParams q = p;
+    if (p.num_tiles < 2u || p.stall_period < 2u) {  // nothing to defer: one ordinary launch
+        q.split_pass = 0;
+        q.stall_period = 0;
+        return launch_csdldf_grid(s, inclusive, q, q.num_tiles, stream);
+    }
+    q.split_pass = 1;
+    hipError_t e = launch_csdldf_grid(s, inclusive, q, split_tile_count(p.num_tiles, p.stall_period, 1u), stream);
+    if (e != hipSuccess) return e;
+    q.split_pass = 2;
+    return launch_csdldf_grid(s, inclusive, q, split_tile_count(p.num_tiles, p.stall_period, 2u), stream);
 }
 
 template <int B, int V, bool I>

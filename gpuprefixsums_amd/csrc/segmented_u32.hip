@@ -327,6 +327,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 #define GPS_SEG_PP_SHAPES(X) \
     X(256, 4)                \
     X(512, 4)                \
+    X(512, 8)                \
     X(1024, 2)               \
     X(1024, 4)               \
     X(1024, 8)

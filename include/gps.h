@@ -54,7 +54,8 @@ typedef struct gps_tuning {
     uint32_t spin_limit;     /* CSDLDF: polls without progress before a fallback (default 4), for the
                                 blocking tile's nearest successor; a waiter k tiles farther waits
                                 (1 + min(k, 7)) x spin_limit polls; at most 65536 */
-    uint32_t stall_period;   /* CSDLDF: stall tiles with index % period == 0 (0 = no emulation, >= 2) */
+    uint32_t stall_period;   /* CSDLDF: stall tiles with index % period == 0 (0 = no emulation, >= 2);
+                                with GPS_SCHEDULE_SPLIT, defer them to a second launch instead */
     uint32_t schedule;       /* CSDL, CSDLDF: GPS_SCHEDULE_* (0 = default) */
 } gps_tuning;
 
@@ -69,6 +70,14 @@ typedef struct gps_tuning {
 /* CONTROL: persistent, with one wave per workgroup dedicated to tickets and
  * lookback; tiles are (block_threads - 64) * vec_per_thread * 4 elements. */
 #define GPS_SCHEDULE_CONTROL 3
+/* SPLIT (CSDLDF only, stall_period P >= 2): the deadlock emulation of the
+ * reference's D3D12 path (GPUPrefixSumsD3D12/EmulatedDeadlock.cpp:97-118,
+ * Shaders/EmulatedDeadlock.hlsl:183-253). Each scan is two launches on the
+ * stream: the first runs every tile except those with index % P == 0, which
+ * have not started when their successors look back, so every such tile with
+ * a successor is reduced and inserted by a fallback; the second launch then
+ * runs the deferred tiles. One workgroup per tile; no in-kernel stalls. */
+#define GPS_SCHEDULE_SPLIT 4
 
 typedef struct gps_handle gps_handle;
 

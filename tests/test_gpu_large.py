@@ -41,6 +41,8 @@ CASES = [
     ("csdldf_stall8", dict(algo=gps.ALGO_CSDLDF, stall_period=8)),
     ("csdldf_pp_stall7", dict(algo=gps.ALGO_CSDLDF, stall_period=7, schedule=gps.SCHEDULE_PERSISTENT)),
     ("csdldf_pp_stall8", dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_PERSISTENT)),
+    ("csdldf_split7", dict(algo=gps.ALGO_CSDLDF, stall_period=7, schedule=gps.SCHEDULE_SPLIT)),
+    ("csdldf_split8", dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_SPLIT)),
 ]
 
 
@@ -67,7 +69,8 @@ def test_exclusive_2p28_full_range(data):
     x, d_in, d_out = data
     want = oracle.scan_u32(x, False)
     for kw in (dict(algo=gps.ALGO_CSDL), dict(algo=gps.ALGO_CSDLDF, stall_period=7),
-               dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_PERSISTENT)):
+               dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_PERSISTENT),
+               dict(algo=gps.ALGO_CSDLDF, stall_period=8, schedule=gps.SCHEDULE_SPLIT)):
         with gps.ScanHandle(max_n=N, **kw) as h:
             h.exclusive(d_in, d_out, N)
             sync()

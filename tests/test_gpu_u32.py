@@ -32,6 +32,10 @@ ALGOS = {
     "csdldf_pp": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_PERSISTENT),
     "csdldf_pp_stall7": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_PERSISTENT, stall_period=7),
     "csdldf_pp_stall8": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_PERSISTENT, stall_period=8),
+    # the D3D12 emulation: tiles % P == 0 deferred to a second launch, so
+    # their successors' fallbacks hit tiles that have not started
+    "csdldf_split2": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_SPLIT, stall_period=2),
+    "csdldf_split8": dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_SPLIT, stall_period=8),
 }
 
 
@@ -316,6 +320,37 @@ def test_emulated_deadlock_counters(period, inclusive, schedule):
         assert c["insertions"] >= stalled
         assert c["fallbacks"] >= c["insertions"]
         assert h.device_error() == 0
+
+
+@pytest.mark.parametrize("period", [2, 7, 8])
+@pytest.mark.parametrize("inclusive", [True, False])
+def test_split_schedule_inserts_every_deferred_tile(period, inclusive):
+    # GPUPrefixSumsD3D12/EmulatedDeadlock.cpp:97-118: the first launch runs
+    # every tile except index % P == 0 and the second launch the deferred
+    # ones. A deferred tile with a successor has not started when that
+    # successor looks back, so a fallback must reduce it from the input and
+    # insert it: insertions >= deferred tiles with a successor, and a fallback
+    # onto tile 0 happens in every launch.
+    n = (1 << 22) + 9
+    x = oracle.gen_u32(31 + period, n, 32)
+    with gps.ScanHandle(gps.ALGO_CSDLDF, n, stall_period=period, spin_limit=4, schedule=gps.SCHEDULE_SPLIT) as h:
+        for rep in range(2):  # the second scan reuses the status words (next epoch)
+            h.reset_counters()
+            check(h, x, inclusive, f"split{period} rep{rep}")
+            c = h.counters()
+            tiles = -(-n // h.tile)
+            deferred = len(range(0, tiles - 1, period))
+            assert c["insertions"] >= deferred, c
+            assert c["stall_timeouts"] == 0
+        assert h.device_error() == 0
+
+
+def test_split_schedule_needs_decoupled_fallback_and_a_period():
+    for kw in (dict(algo=gps.ALGO_CSDL, schedule=gps.SCHEDULE_SPLIT),
+               dict(algo=gps.ALGO_CSDLDF, schedule=gps.SCHEDULE_SPLIT),
+               dict(algo=gps.ALGO_RTS, schedule=gps.SCHEDULE_SPLIT)):
+        with pytest.raises(gps.GpsError, match="split"):
+            gps.ScanHandle(max_n=4096, **kw)
 
 
 @pytest.mark.parametrize("spin_limit", [1, 32])

@@ -189,6 +189,85 @@ def run_persistent(mem, agg, epoch, period, spin_limit, grid, rng, stats):
     return result
 
 
+def run_split(mem, agg, epoch, period, spin_limit, resident, rng, stats):
+    """The split schedule (GPS_SCHEDULE_SPLIT, the D3D12 deadlock emulation):
+    a first launch of every tile with index % period != 0, in order, then a
+    second launch of the deferred tiles under the same epoch. Returns the
+    prefixes and, as of the end of the first launch, the deferred tiles whose
+    word a fallback had inserted (all of them had not started)."""
+    num_tiles = len(agg)
+    result = [None] * num_tiles
+    first = [t for t in range(num_tiles) if t % period]
+    second = [t for t in range(num_tiles) if t % period == 0]
+    inserted_before_start = []
+    for launch in (first, second):
+        progs, active, i, steps = {}, [], 0, 0
+        while i < len(launch) or active:
+            while len(active) < resident and i < len(launch):  # dispatch in ticket order
+                t = launch[i]
+                progs[t] = tile_program(mem, t, agg, epoch, num_tiles, 0, spin_limit, stats, result)
+                active.append(t)
+                i += 1
+            t = rng.choice(active)
+            try:
+                next(progs[t])
+            except StopIteration:
+                active.remove(t)
+            steps += 1
+            assert steps < 5_000_000, "protocol did not terminate"
+        if launch is first:
+            inserted_before_start = [t for t in second if flag_of(mem.status[t], epoch)]
+    return result, inserted_before_start
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_split_schedule_under_random_interleavings(seed):
+    # Deferred tiles never run during the first launch, so the fallback must
+    # reduce and insert every one of them that has a successor; the second
+    # launch then completes them. Every prefix is exact.
+    rng = random.Random(700 + seed)
+    num_tiles = rng.choice([2, 3, 9, 40, 130])
+    mem = Memory(200)
+    for launch in range(3):
+        epoch = launch + 1
+        agg = [rng.getrandbits(32) for _ in range(num_tiles)]
+        period = rng.choice([2, 3, 7, 8])
+        stats = Counter()
+        res, inserted = run_split(mem, agg, epoch, period, spin_limit=rng.choice([1, 2, 4]),
+                                  resident=rng.choice([2, 5, 17]), rng=rng, stats=stats)
+        acc = 0
+        for t in range(num_tiles):
+            assert res[t] == acc, (seed, launch, t)
+            acc = (acc + agg[t]) & MASK32
+        deferred_with_successor = [t for t in range(0, num_tiles, period) if t + 1 < num_tiles]
+        assert inserted == deferred_with_successor
+        assert stats["cas_win"] >= len(deferred_with_successor)
+        if 0 in deferred_with_successor:
+            assert stats["fallback_tile0"] > 0
+        assert all(flag_of(mem.status[t], epoch) == INCL for t in range(num_tiles))
+
+
+def test_split_tile_mapping_matches_the_device_formula():
+    # split_tile / split_tile_count in device_common.hpp: ticket q of the
+    # first launch is the q-th tile with index % P != 0, of the second q * P.
+    for num_tiles in (2, 3, 7, 8, 9, 64, 1000):
+        for P in (2, 3, 7, 8):
+            deferred = (num_tiles + P - 1) // P
+            first = [q + q // (P - 1) + 1 for q in range(num_tiles - deferred)]
+            second = [q * P for q in range(deferred)]
+            assert first == [t for t in range(num_tiles) if t % P]
+            assert second == [t for t in range(num_tiles) if t % P == 0]
+
+
+def test_prev_epoch_inverts_next_epoch():
+    def prev_epoch(e):  # device_common.hpp, on masked epochs
+        p = (e - 1) & EPOCH_MASK
+        return EPOCH_MASK if p == 0 else p
+    for raw in (1, 2, 5, EPOCH_MASK - 1, EPOCH_MASK, EPOCH_MASK + 2, (7 << 30) + 3):
+        nxt = next_epoch(raw)
+        assert prev_epoch(nxt & EPOCH_MASK) == raw & EPOCH_MASK
+
+
 @pytest.mark.parametrize("seed", range(12))
 def test_protocol_under_random_interleavings(seed):
     rng = random.Random(seed)
