@@ -14,7 +14,10 @@ What each function restates (reference files relative to the reference root):
   segmented_scan_u32        per-segment serial loop (no reference counterpart)
   scan_f32_f64 / bf16       FP64 Neumaier-compensated running sum (no counterpart)
   segmented_scan_fp_f64     the FP64 scans restarted at every segment offset
-  generators                counter-based splitmix64 streams (no counterpart)
+  generators                counter-based splitmix64 streams (no counterpart); the
+                            reference's D3D12 and Unity test generators,
+                            GPUPrefixSumsD3D12/Shaders/Utility.hlsl:37-66 and
+                            GPUPrefixSumsUnity/Shaders/Utility.compute:50-72
 
 How each is pinned (tests/test_oracle.py): closed forms, brute force with
 numpy uint64 cumsum, invariants (excl[0] = 0, excl[i+1] = incl[i], last =
@@ -93,6 +96,8 @@ def lib():
     L.orc_gen_lognormal_offsets.restype = _u64
     L.orc_gen_d3d12_taus.argtypes = [ctypes.c_uint32, _u32p, _u64]
     L.orc_gen_d3d12_taus.restype = None
+    L.orc_gen_unity_taus.argtypes = [ctypes.c_uint32, _u32p, _u64]
+    L.orc_gen_unity_taus.restype = None
     _lib = L
     return L
 
@@ -247,6 +252,16 @@ def gen_lognormal_offsets(seed: int, num_segments: int, median: float = 4096.0, 
     off = np.empty(num_segments + 1, np.uint64)
     lib().orc_gen_lognormal_offsets(seed, num_segments, median, sigma, lo, hi, off)
     return off
+
+
+def gen_unity_taus(seed: int, n: int) -> np.ndarray:
+    """Inputs of the reference's Unity random tests (bytes of a per-thread
+    hybrid Tausworthe stream, values in [0, 255]); the reference's own tests
+    run it with seed 0 (GPUPrefixSumsUnity/Shaders/Utility.compute:50-72)."""
+    out = np.empty(n, dtype=np.uint32)
+    if n:
+        lib().orc_gen_unity_taus(seed & 0xFFFFFFFF, out, n)
+    return out
 
 
 def gen_d3d12_taus(seed: int, n: int) -> np.ndarray:

@@ -274,3 +274,24 @@ void orc_gen_d3d12_taus(uint32_t seed, uint32_t* out, uint64_t n) {
         out[i] = (a ^ b ^ c ^ d) & 1023u;
     }
 }
+
+void orc_gen_unity_taus(uint32_t seed, uint32_t* out, uint64_t n) {
+    const uint64_t nvec = (n + 3u) / 4u;  /* uint4 elements, VectorizedSize(n) */
+    for (uint32_t tid = 0; tid < 65536u && tid < nvec; ++tid) {
+        uint32_t a = (tid << 2) * seed;
+        uint32_t b = ((tid << 2) + 1u) * seed;
+        uint32_t c = ((tid << 2) + 2u) * seed;
+        uint32_t d = ((tid << 2) + 3u) * seed;
+        for (uint64_t v = tid; v < nvec; v += 65536u) {
+            a = ((a & 0xFFFFFFFEu) << 12) ^ (((a << 13) ^ a) >> 19);
+            b = ((b & 0xFFFFFFF8u) << 4) ^ (((b << 2) ^ b) >> 25);
+            c = ((c & 0xFFFFFFF0u) << 17) ^ (((c << 3) ^ c) >> 11);
+            d = d * 1664525u + 1013904223u;
+            const uint32_t t = a ^ b ^ c ^ d;
+            for (uint32_t k = 0; k < 4u; ++k) {
+                const uint64_t i = 4u * v + k;
+                if (i < n) out[i] = (t >> (8u * k)) & 255u;
+            }
+        }
+    }
+}

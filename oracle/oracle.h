@@ -112,6 +112,17 @@ uint64_t orc_gen_lognormal_offsets(uint64_t seed, uint64_t num_segments, double 
  * advances its state once per thread, not per element). */
 void orc_gen_d3d12_taus(uint32_t seed, uint32_t* out, uint64_t n);
 
+/* The Unity test generator: GPUPrefixSumsUnity/Shaders/Utility.compute:50-72
+ * (InitRandom). The same hybrid Tausworthe steps, but the 65536 threads
+ * (256 groups of 256) keep their state and step it once per uint4 element
+ * they visit (uint4 element v belongs to thread v mod 65536), and the 32-bit
+ * output is split into four bytes, low byte first, one per scan element.
+ * Elements 4v..4v+3 of n (the last uint4 may be partial). The reference's
+ * test harness never sets the seed (GPUPrefixSumsUnity/Runtime/Tests.cs
+ * sets no "e_seed"), so its effective seed is 0: only the LCG component is
+ * nonzero and every thread yields the same sequence. */
+void orc_gen_unity_taus(uint32_t seed, uint32_t* out, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -186,6 +186,27 @@ def test_d3d12_taus_lcg_part():
     assert np.array_equal(z[:10], z[65536:])
 
 
+def test_unity_taus_known_values():
+    # Seed 0 (what the reference's Unity tests run): the Tausworthe components
+    # stay 0 and every thread's LCG starts at 0, so uint4 element v holds the
+    # bytes, low first, of the (v // 65536 + 1)-th output of the Numerical
+    # Recipes LCG from 0: 1013904223 = 0x3C6EF35F, then 1196435762 = 0x47502932.
+    x = oracle.gen_unity_taus(0, 4 * 65536 + 8)
+    assert list(x[:4]) == [0x5F, 0xF3, 0x6E, 0x3C]
+    assert np.all(x[:4 * 65536].reshape(-1, 4) == x[:4])  # every thread alike in the first round
+    assert list(x[4 * 65536:4 * 65536 + 4]) == [0x32, 0x29, 0x50, 0x47]
+    # Seed 1, thread 0: states (0, 1, 2, 3); one Tausworthe step maps 0, 1
+    # and 2 to 0 (the shifted-out bits), so the output is the LCG of 3:
+    # 3 * 1664525 + 1013904223 = 1018897798 = 0x3CBB2586 (bytes low first).
+    y = oracle.gen_unity_taus(1, 8)
+    assert list(y[:4]) == [0x86, 0x25, 0xBB, 0x3C]
+    # per-thread state persists: thread 0's second element differs from its first
+    z = oracle.gen_unity_taus(12345, 4 * 65536 + 4)
+    assert not np.array_equal(z[:4], z[4 * 65536:]) and z.max() <= 255
+    # partial last uint4: the first n values of the longer stream
+    assert np.array_equal(oracle.gen_unity_taus(9, 4099), oracle.gen_unity_taus(9, 4100)[:4099])
+
+
 # ---- floating point --------------------------------------------------------
 
 def test_bf16_roundtrip_and_rne():
