@@ -831,6 +831,184 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent FP scan over lane-blocked runs (GPS_SCHEDULE_PERSISTENT with
+// pair carries; the default above 2^24 elements).
+//
+// scan_fp_pp pays one pair wave scan per uint4 vector: VEC scans of 64 lane
+// totals per wave run, about 60 VALU instructions each, most of the pair
+// path's ~38 VALU instructions per element. Here each wave transposes its
+// run through its own slice of LDS after the coalesced loads, so lane l holds
+// the 4*VEC consecutive elements 4*VEC*l .. 4*VEC*l + 4*VEC - 1; the lane
+// sums them serially (TwoSum pairs), one pair wave scan per run gives every
+// lane its carry, the outputs are recomputed serially from that carry, and
+// the outputs are transposed back for coalesced stores. Per element that is
+// one TwoSum add for the total and one for the output, plus 1/(4*VEC) of a
+// wave scan; the two transposes are four LDS instructions per lane-vector.
+//
+// LDS slice per wave: 64 rows (one per lane) of 4*VEC floats padded by 4, so
+// that eight consecutive lanes' 16-byte accesses, row-wise or column-wise,
+// fall in distinct bank groups. Each wave reads only what it wrote; LDS
+// instructions of one wave complete in order, and a wavefront-scope fence
+// keeps the compiler from reordering the cross-lane write/read pairs.
+// ---------------------------------------------------------------------------
+template <int VEC>
+struct BlockedSlice {
+    static constexpr uint32_t kRow = 4u * VEC + 4u;  // floats per padded row
+    static constexpr uint32_t kFloats = 64u * kRow;
+};
+
+// Striped (lane l, vector k) <-> blocked (lane l, vector q) through the
+// wave's LDS slice. Striped element of (k, l, c): 256k + 4l + c; blocked
+// element of (l, q, c): 4*VEC*l + 4q + c.
+template <int VEC>
+__device__ __forceinline__ void striped_to_blocked(f32x4 (&x)[VEC], float* slice) {
+    using S = BlockedSlice<VEC>;
+    const uint32_t l = lane_id();
+    constexpr uint32_t kRowsPerVec = 256u / (4u * VEC);  // rows covered by one striped vector
+    constexpr uint32_t kLanesPerRow = VEC;                // striped lanes that share a row
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        const uint32_t row = k * kRowsPerVec + l / kLanesPerRow;
+        *reinterpret_cast<f32x4*>(slice + row * S::kRow + 4u * (l % kLanesPerRow)) = x[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) x[q] = *reinterpret_cast<const f32x4*>(slice + l * S::kRow + 4u * q);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int VEC>
+__device__ __forceinline__ void blocked_to_striped(f32x4 (&x)[VEC], float* slice) {
+    using S = BlockedSlice<VEC>;
+    const uint32_t l = lane_id();
+    constexpr uint32_t kRowsPerVec = 256u / (4u * VEC);
+    constexpr uint32_t kLanesPerRow = VEC;
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) *reinterpret_cast<f32x4*>(slice + l * S::kRow + 4u * q) = x[q];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) {
+        const uint32_t row = k * kRowsPerVec + l / kLanesPerRow;
+        x[k] = *reinterpret_cast<const f32x4*>(slice + row * S::kRow + 4u * (l % kLanesPerRow));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int BLOCK, int VEC, bool BF16, class Ops>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void scan_fp_bl(FpParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    using T = typename Ops::T;
+    static_assert(VEC == 2 || VEC == 4 || VEC == 8, "blocked runs need 256 / (4 VEC) whole rows per vector");
+    __shared__ float s_slices[C::kWaves * BlockedSlice<VEC>::kFloats];
+    __shared__ T s_w[C::kWaves];
+    __shared__ T s_p;
+    __shared__ uint32_t s_ticket[3];
+    const uint32_t wave = wave_index();
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+    float* slice = s_slices + wave * BlockedSlice<VEC>::kFloats;
+
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, total_tickets);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    __syncthreads();
+    uint32_t tile = s_ticket[0];
+    const uint32_t epoch = s_ticket[1];
+    if (tile >= p.num_tiles) return;
+
+    f32x4 x[VEC], y[VEC];
+    load_fp<BF16, VEC>(x, p.in, thread_base<BLOCK, VEC>(tile), p.n, p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n));
+
+    while (true) {
+        // Lane total of the lane's consecutive run, then one pair wave scan.
+        striped_to_blocked<VEC>(x, slice);
+        T t = {x[0].x, 0.f};
+        t = Ops::add1(t, x[0].y);
+        t = Ops::add1(t, x[0].z);
+        t = Ops::add1(t, x[0].w);
+#pragma unroll
+        for (int q = 1; q < VEC; ++q) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) t = Ops::add1(t, x[q][c]);
+        }
+        const T sc = Ops::scan(t);
+        const T lane_carry = Ops::shr1(sc);
+        if (lane == 0) s_w[wave] = Ops::lane(sc, 63);
+        __syncthreads();
+        T wave_pre, tile_tot;
+        {
+            const T v = lane < C::kWaves ? s_w[lane] : Ops::zero();
+            const T ws = Ops::scan(v);
+            wave_pre = Ops::lane(Ops::shr1(ws), (int)wave);
+            tile_tot = Ops::lane(ws, C::kWaves - 1);
+        }
+        if (threadIdx.x == 0) {
+            Ops::publish(p.status, tile, tile_tot, tile ? kFlagAgg : kFlagIncl, epoch);
+            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+        }
+        __syncthreads();
+        const uint32_t next = s_ticket[2];
+        const bool has_next = next < p.num_tiles;
+
+        LookbackW2<Ops> lb;
+        if (wave == 0 && tile != 0) {
+            lb.start(tile);
+            lb.issue(p.status, epoch);
+        }
+        if (has_next)
+            load_fp<BF16, VEC>(y, p.in, thread_base<BLOCK, VEC>(next), p.n,
+                               p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n));
+        if (wave == 0) {
+            T prefix = Ops::zero();
+            if (tile != 0) {
+                while (!lb.consume(epoch, p.err)) lb.issue(p.status, epoch);
+                prefix = lb.prefix;
+                if (lane == 0) Ops::publish(p.status, tile, Ops::add(prefix, tile_tot), kFlagIncl, epoch);
+            }
+            if (lane == 0) s_p = prefix;
+        }
+        __syncthreads();
+        // Outputs: the lane's carry, then its run serially, each rounded once.
+        T acc = Ops::add(Ops::add(s_p, wave_pre), lane_carry);
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            f32x4 o;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                acc = Ops::add1(acc, x[q][c]);
+                o[c] = Ops::round(acc);
+            }
+            x[q] = o;
+        }
+        blocked_to_striped<VEC>(x, slice);
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n)) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+                __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                    if (i < p.n) p.out[i] = x[k][c];
+                }
+        }
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) x[k] = y[k];
+        tile = next;
+    }
+}
+
 #define GPS_FP_SHAPES(X) \
     X(256, 4)            \
     X(512, 8)            \
@@ -854,9 +1032,20 @@ static hipError_t launch_fp_pp(const FpParams& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
+template <int B, int V, bool BF16, class Ops>
+static hipError_t launch_fp_bl(const FpParams& p, hipStream_t stream) {
+    const uint32_t resident = resident_workgroups<scan_fp_bl<B, V, BF16, Ops>, B>();
+    if (resident == 0) return hipErrorInvalidDevice;
+    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    hipLaunchKernelGGL((scan_fp_bl<B, V, BF16, Ops>), dim3(grid), dim3(B), 0, stream, p);
+    return hipGetLastError();
+}
+
+// Pair carries run the lane-blocked kernel; plain carries, with little
+// arithmetic to save, the striped one.
 template <int B, int V, bool BF16>
 static hipError_t launch_fp_pp_carry(int carry, const FpParams& p, hipStream_t stream) {
-    if (carry == kFpPair) return launch_fp_pp<B, V, BF16, PairCarry<true>>(p, stream);
+    if (carry == kFpPair) return launch_fp_bl<B, V, BF16, PairCarry<true>>(p, stream);
     return launch_fp_pp<B, V, BF16, PairCarry<false>>(p, stream);
 }
 
