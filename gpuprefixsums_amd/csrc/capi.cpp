@@ -72,26 +72,6 @@ gps::TileShape default_shape(int algo) {
     }
 }
 
-// Shape and schedule of a default CSDL handle for an n-element scan,
-// measured on MI355X (back-to-back launches): below 2^24 elements one
-// workgroup per tile beats the persistent grid, which has too few tiles per
-// workgroup to pay for its ramp; below 2^22 smaller tiles also shorten the
-// lookback chain. 2^16: 3.6 against 5.5 us; 2^20: 6.1 against 7.4 us; 2^23:
-// 14.4 against 15.4 us; from 2^24 up the persistent 1024 x 8 kernel wins
-// (26.7 against 28.9 us at 2^24).
-void size_adaptive_config(uint64_t n, gps::TileShape& shape, uint32_t& schedule) {
-    if (n <= (1ull << 18)) {
-        shape = {1024, 2};
-        schedule = GPS_SCHEDULE_TILE;
-    } else if (n <= (1ull << 21)) {
-        shape = {1024, 4};
-        schedule = GPS_SCHEDULE_TILE;
-    } else if (n < (1ull << 24)) {
-        shape = {1024, 8};
-        schedule = GPS_SCHEDULE_TILE;
-    }
-}
-
 // Elements per tile of the plain-scan kernel for a shape and schedule.
 uint32_t kernel_tile(gps::TileShape shape, uint32_t schedule) {
     return schedule == GPS_SCHEDULE_CONTROL ? gps::cw_tile_elems(shape) : shape.tile();
@@ -143,7 +123,7 @@ int scan_u32(gps_handle* h, const uint32_t* in, uint32_t* out, uint64_t n, bool 
     }
     gps::TileShape shape = h->shape;
     uint32_t schedule = h->schedule;
-    if (h->size_adaptive) size_adaptive_config(n, shape, schedule);
+    if (h->size_adaptive) gps::size_adaptive_config(n, shape, schedule);
     if ((carry_in || carry_out) &&
         (h->algo != GPS_ALGO_CSDL || schedule == GPS_SCHEDULE_CONTROL))
         return fail(GPS_ERR_UNSUPPORTED, "carries are offered by the CSDL tile and persistent schedules");

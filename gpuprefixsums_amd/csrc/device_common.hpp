@@ -60,22 +60,29 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) { return readlane(wave_
 // travels inside the word, so no other memory needs ordering:
 //   bits  0..31  value: the tile aggregate (FLAG_AGG) or the inclusive prefix
 //                through this tile (FLAG_INCL), full 32 bits, wraps mod 2^32
-//   bits 32..33  flag
+//   bits 32..33  flag: 1 aggregate, 2 inclusive prefix, 3 claimed (the
+//                decoupled fallback's "being reduced by a waiter", no value)
 //   bits 34..63  epoch of the launch that wrote it (30 bits)
 // A word whose epoch differs from the current launch's is "not ready"; the
 // array is therefore never cleared between launches.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kFlagAgg = 1;
 constexpr uint64_t kFlagIncl = 2;
+constexpr uint64_t kFlagClaim = 3;
 constexpr uint32_t kEpochBits = 30;
 constexpr uint32_t kEpochMask = (1u << kEpochBits) - 1u;
 
 __device__ __forceinline__ uint64_t make_status(uint32_t value, uint64_t flag, uint32_t epoch) {
     return (uint64_t)value | (flag << 32) | ((uint64_t)epoch << 34);
 }
-// Flag of a word for this launch's epoch: 0 (not ready), kFlagAgg or kFlagIncl.
+// Flag of a word for this launch's epoch: 0 (not ready: stale, or claimed
+// but not yet reduced), kFlagAgg or kFlagIncl.
 __device__ __forceinline__ uint32_t status_flag(uint64_t w, uint32_t epoch) {
-    return ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
+    const uint32_t f = (uint32_t)((w >> 32) & 3u);
+    return ((uint32_t)(w >> 34) == epoch && f != (uint32_t)kFlagClaim) ? f : 0u;
+}
+__device__ __forceinline__ bool status_claimed(uint64_t w, uint32_t epoch) {
+    return (uint32_t)(w >> 34) == epoch && (uint32_t)((w >> 32) & 3u) == (uint32_t)kFlagClaim;
 }
 
 __device__ __forceinline__ uint64_t atomic_load_agent(const uint64_t* p) {

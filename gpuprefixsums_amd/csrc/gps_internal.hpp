@@ -65,6 +65,18 @@ uint32_t resident_workgroups() {
     return r.workgroups;
 }
 
+// Compute units of the current device (queried once per process; 0 if the
+// query fails).
+inline uint32_t device_cus() {
+    static const uint32_t cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0u;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0u;
+        return (uint32_t)(n > 0 ? n : 0);
+    }();
+    return cus;
+}
+
 struct TileShape {
     uint32_t block;
     uint32_t vec;
@@ -72,6 +84,29 @@ struct TileShape {
 };
 
 bool shape_supported(TileShape s);
+
+// Shape and schedule of a default CSDL handle for an n-element scan (the
+// schedule constants of gps.h: 1 = one workgroup per tile, 2 = persistent),
+// measured on MI355X (back-to-back launches): below 2^24 elements one
+// workgroup per tile beats the persistent grid, which has too few tiles per
+// workgroup to pay for its ramp; below 2^22 smaller tiles also shorten the
+// lookback chain. 2^16: 3.6 against 5.5 us; 2^20: 6.1 against 7.4 us; 2^23:
+// 14.4 against 15.4 us; from 2^24 up the persistent 1024 x 8 kernel wins
+// (26.7 against 28.9 us at 2^24). Leaves shape and schedule unchanged from
+// 2^24 up (the persistent 1024 x 8 default).
+inline void size_adaptive_config(uint64_t n, TileShape& shape, uint32_t& schedule) {
+    constexpr uint32_t kTile = 1;  // GPS_SCHEDULE_TILE
+    if (n <= (1ull << 18)) {
+        shape = {1024, 2};
+        schedule = kTile;
+    } else if (n <= (1ull << 21)) {
+        shape = {1024, 4};
+        schedule = kTile;
+    } else if (n < (1ull << 24)) {
+        shape = {1024, 8};
+        schedule = kTile;
+    }
+}
 
 // Each launcher enqueues exactly one kernel (RTS: three) on `stream` and
 // returns the HIP launch status.

@@ -130,11 +130,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES
 // lowest tile without a published aggregate belongs to a workgroup whose
 // current tile is lower still and already has every predecessor published.
 // ---------------------------------------------------------------------------
+// Tile layout of the persistent scan: the wave-run layout of the other
+// kernels, or, with GPS_PP_SLAB=1, slabs (tile_u32.hpp). At 1024 x 8 and
+// 2^28 elements slabs measured 6036 against 6123 GB/s; a ticketed persistent
+// copy of the scan's shape gains from slabs (6134 against 6016), so the
+// lookback and scan, not the access pattern, decide it for the scan.
+#ifndef GPS_PP_SLAB
+#define GPS_PP_SLAB 0
+#endif
+
 template <int BLOCK, int VEC, bool INCL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(VEC >= 8 ? 4 : GPS_WAVES_PER_EU))) void
 csdl_pp_u32(ScanParams p) {
     using C = TileCfg<BLOCK, VEC>;
-    __shared__ uint32_t s_wave[C::kWaves];
+    constexpr bool kSlab = GPS_PP_SLAB != 0;
+    __shared__ uint32_t s_wave[kSlab ? C::kWaves * VEC : C::kWaves];
     __shared__ uint32_t s_ticket[3];
     __shared__ uint32_t s_prefix;
     __shared__ uint32_t s_carry;
@@ -153,19 +163,50 @@ csdl_pp_u32(ScanParams p) {
     if (tile >= p.num_tiles) return;
     if (threadIdx.x == 0) atomic_add_counter(p.occupancy, 1ull);
 
+    // Loads tile t into d (full tiles with dwordx4 nontemporal loads).
+    auto load_tile = [&](u32x4(&d)[VEC], uint32_t t) {
+        const bool full = p.vec_ok && tile_is_full<BLOCK, VEC>(t, p.n);
+        if constexpr (kSlab) {
+            const uint64_t base = slab_thread_base<BLOCK, VEC>(t);
+            if (full)
+                load_slab<BLOCK, VEC>(d, p.in + base);
+            else
+                load_slab_guarded<BLOCK, VEC>(d, p.in, base, p.n);
+        } else {
+            const uint64_t base = thread_base<BLOCK, VEC>(t);
+            if (full)
+                load_vec<VEC>(d, p.in + base);
+            else
+                load_guarded<VEC>(d, p.in, base, p.n);
+        }
+    };
+
     u32x4 a[VEC], b[VEC];
-    const uint64_t base = thread_base<BLOCK, VEC>(tile);
-    if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
-        load_vec<VEC>(a, p.in + base);
-    else
-        load_guarded<VEC>(a, p.in, base, p.n);
+    load_tile(a, tile);
 
     while (true) {
-        const uint32_t run = wave_scan_run<VEC, INCL>(a);
-        if (lane == 0) s_wave[wave] = run;
-        __syncthreads();
-        uint32_t wave_prefix, tile_total;
-        combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
+        // Scan the tile in registers; part_prefix[k] is the exclusive prefix
+        // inside the tile of this wave's vector k (the same for every k in
+        // the wave-run layout).
+        uint32_t part_prefix[VEC], tile_total;
+        if constexpr (kSlab) {
+            uint32_t tot[VEC];
+            slab_scan_parts<VEC, INCL>(a, tot);
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) s_wave[wave * VEC + k] = tot[k];
+            }
+            __syncthreads();
+            combine_slabs<C::kWaves, VEC>(s_wave, wave, part_prefix, tile_total);
+        } else {
+            const uint32_t run = wave_scan_run<VEC, INCL>(a);
+            if (lane == 0) s_wave[wave] = run;
+            __syncthreads();
+            uint32_t wave_prefix;
+            combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) part_prefix[k] = wave_prefix;
+        }
         if (threadIdx.x == 0) {
             // Publish first, then take the next ticket: ticket order then
             // follows publication order, so lookbacks find contiguous ready
@@ -188,13 +229,7 @@ csdl_pp_u32(ScanParams p) {
             lb.start(tile);
             lb.issue(p.status, epoch);
         }
-        if (has_next) {
-            const uint64_t next_base = thread_base<BLOCK, VEC>(next);
-            if (p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n))
-                load_vec<VEC>(b, p.in + next_base);
-            else
-                load_guarded<VEC>(b, p.in, next_base, p.n);
-        }
+        if (has_next) load_tile(b, next);
         if (wave == 0) {
             uint32_t prefix = tile == 0 ? s_carry : 0u;  // tile 0 starts from the carry-in
             if (tile != 0) {
@@ -208,12 +243,24 @@ csdl_pp_u32(ScanParams p) {
             }
         }
         __syncthreads();
-        const uint32_t add = s_prefix + wave_prefix;
-        const uint64_t base = thread_base<BLOCK, VEC>(tile);
-        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
-            store_vec<VEC>(p.out + base, a, add);
-        else
-            store_guarded<VEC>(p.out, a, add, base, p.n);
+        const bool full = p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n);
+        if constexpr (kSlab) {
+            uint32_t add[VEC];
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) add[k] = s_prefix + part_prefix[k];
+            const uint64_t base = slab_thread_base<BLOCK, VEC>(tile);
+            if (full)
+                store_slab<BLOCK, VEC>(p.out + base, a, add);
+            else
+                store_slab_guarded<BLOCK, VEC>(p.out, a, add, base, p.n);
+        } else {
+            const uint32_t add = s_prefix + part_prefix[0];
+            const uint64_t base = thread_base<BLOCK, VEC>(tile);
+            if (full)
+                store_vec<VEC>(p.out + base, a, add);
+            else
+                store_guarded<VEC>(p.out, a, add, base, p.n);
+        }
         if (!has_next) break;
 #pragma unroll
         for (int k = 0; k < VEC; ++k) a[k] = b[k];
@@ -372,11 +419,14 @@ csdl_cw_u32(ScanParams p) {
 // CSDLDF: lookback with decoupled fallback.
 //
 // Wave 0 polls the status window; after spin_limit consecutive polls without
-// progress it hands the nearest not-ready tile index j to the workgroup, which
-// reduces tile j's input from memory. Lane 0 then CAS-inserts that aggregate
-// into j's word (inclusive if j == 0) if the word is still stale; on a lost
-// race it uses the word it found. Either way the lookback continues below j,
-// or ends at an inclusive word.
+// progress it claims the nearest not-ready tile j (CAS of the stale word to a
+// claim word) and hands j to the workgroup, which reduces tile j's input from
+// memory. Lane 0 then CAS-inserts that aggregate over the claim (inclusive if
+// j == 0); on a lost race (the owner published meanwhile) it uses the word it
+// found. Either way the lookback continues below j, or ends at an inclusive
+// word. A waiter that finds j already claimed keeps polling, so one reduction
+// serves all waiters; after kClaimPatience polls on a claim it reduces j
+// anyway (every waiter still falls back within a bounded number of polls).
 //
 // Emulated deadlock: with stall_period > 0, tiles with tile % period == 0 that
 // have a successor hold back their own publication until some successor has
@@ -384,10 +434,10 @@ csdl_cw_u32(ScanParams p) {
 // !(idx & 7) (period 8) and also stalls a last tile, which would never be
 // released; here the last tile never stalls.
 //
-// In-place safety (in == out): a fallback reads tile j's input. Its CAS only
-// wins while j has not published, and j publishes with a returning exchange
-// that it waits on before its first output store, so a winning reduction
-// never saw j's output.
+// In-place safety (in == out): a fallback reads tile j's input. Its final CAS
+// only wins while j has not published (a publication replaces a claim word
+// too), and j publishes with a returning exchange that it waits on before its
+// first output store, so a winning reduction never saw j's output.
 // ---------------------------------------------------------------------------
 
 // Polls a waiter makes on one stale word before it reduces that tile itself:
@@ -398,6 +448,38 @@ csdl_cw_u32(ScanParams p) {
 __device__ __forceinline__ uint32_t fallback_patience(uint32_t spin_limit, uint32_t tile, int64_t top) {
     const int64_t gap = (int64_t)tile - 1 - top;
     return spin_limit * (1u + (uint32_t)(gap < 7 ? gap : 7));
+}
+
+// Polls a waiter makes on a claimed word before it reduces that tile anyway.
+constexpr uint32_t kClaimPatience = 64;
+
+// Called by all of wave 0 when its patience on the stale word w0 of tile
+// `top` has run out (w0 is lane 0's, i.e. the nearest not-ready tile's,
+// word). A waiter claims the tile before reducing it, so one redundant
+// reduction serves every waiter: it CASes w0 to a claim word (flag 3).
+//   - claim won: returns true with `expect` = the claim word, which the
+//     final insertion CASes over (it fails only if the owner published);
+//   - w0 is already a claim: keeps waiting (returns false) for up to
+//     kClaimPatience polls, then returns true with `expect` = w0, so a
+//     waiter still falls back after a bounded number of polls;
+//   - CAS lost to a publication or another claim: returns false (poll on).
+// The caller resets its spin count when false is returned.
+__device__ __forceinline__ bool claim_for_fallback(uint64_t* status, int64_t top, uint64_t w0, uint32_t epoch,
+                                                   uint32_t& claim_waits, uint64_t& expect) {
+    if (status_claimed(w0, epoch)) {
+        if (++claim_waits < kClaimPatience) return false;
+        claim_waits = 0;
+        expect = w0;
+        return true;
+    }
+    const uint64_t claim = make_status(0u, kFlagClaim, epoch);
+    uint64_t found = w0;
+    if (lane_id() == 0) found = atomic_cas_agent(status + kSlotWords * (uint64_t)top, w0, claim);
+    found = ((uint64_t)readlane((uint32_t)(found >> 32), 0) << 32) | readlane((uint32_t)found, 0);
+    if (found != w0) return false;
+    claim_waits = 0;
+    expect = claim;
+    return true;
 }
 
 template <int BLOCK, int VEC, bool INCL>
@@ -435,7 +517,7 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
 
     if (tile != 0) {
         // Lookback state, meaningful in wave 0 only (uniform there).
-        uint32_t prefix = 0, spins = 0;
+        uint32_t prefix = 0, spins = 0, claim_waits = 0;
         int64_t top = (int64_t)tile - 1;
         uint64_t stale = 0;
         unsigned long long c_polls = 0, c_fallbacks = 0, c_inserts = 0;
@@ -464,8 +546,10 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
                         top -= run;
                         spins = 0;
                     } else if (++spins >= fallback_patience(p.spin_limit, tile, top)) {
-                        stale = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
-                        break;
+                        const uint64_t w0 = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
+                        if (claim_for_fallback(p.status, top, w0, epoch, claim_waits, stale)) break;
+                        spins = 0;
+                        __builtin_amdgcn_s_sleep(1);
                     } else {
                         __builtin_amdgcn_s_sleep(1);
                     }
@@ -651,7 +735,7 @@ csdldf_pp_u32(ScanParams p) {
         // Lookback state (wave 0; uniform there). The first window is issued
         // ahead of the prefetch so the in-order vmcnt does not make its use
         // wait for tile i+1's loads.
-        uint32_t prefix = 0, spins = 0;
+        uint32_t prefix = 0, spins = 0, claim_waits = 0;
         int32_t top = (int32_t)tile - 1;  // 32-bit: keeps 1024 x 4 within 64 VGPRs
         uint64_t w = 0, stale = 0;
         if (wave == 0 && tile != 0) {
@@ -687,8 +771,10 @@ csdldf_pp_u32(ScanParams p) {
                         top -= run_len;
                         spins = 0;
                     } else if (++spins >= fallback_patience(p.spin_limit, tile, top)) {
-                        stale = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
-                        break;
+                        const uint64_t w0 = ((uint64_t)readlane((uint32_t)(w >> 32), 0) << 32) | readlane((uint32_t)w, 0);
+                        if (claim_for_fallback(p.status, top, w0, epoch, claim_waits, stale)) break;
+                        spins = 0;
+                        __builtin_amdgcn_s_sleep(1);
                     } else {
                         __builtin_amdgcn_s_sleep(1);
                     }
@@ -1008,10 +1094,16 @@ hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hip
     return hipErrorInvalidValue;
 }
 
-// Persistent grid: as many workgroups as fit on the device at once.
+// Persistent grid: as many workgroups as fit on the device at once, at most
+// GPS_PP_WG_PER_CU per CU.
+#ifndef GPS_PP_WG_PER_CU
+#define GPS_PP_WG_PER_CU 1
+#endif
 template <int B, int V, bool I>
 static uint32_t pp_residency() {
-    return resident_workgroups<csdl_pp_u32<B, V, I>, B>();
+    const uint32_t r = resident_workgroups<csdl_pp_u32<B, V, I>, B>();
+    const uint32_t cap = device_cus() * GPS_PP_WG_PER_CU;
+    return cap && r > cap ? cap : r;
 }
 
 template <int B, int V, bool I>

@@ -20,8 +20,10 @@ from collections import Counter
 import pytest
 
 MASK32 = 0xFFFFFFFF
-AGG, INCL = 1, 2
+AGG, INCL, CLAIM = 1, 2, 3
 EPOCH_MASK = (1 << 30) - 1
+CLAIM_PATIENCE = 64  # kClaimPatience in scan_u32.hip
+REDUCTION_STEPS = 4  # a redundant reduction is long next to one status access
 
 
 def make_status(value, flag, epoch):
@@ -29,7 +31,13 @@ def make_status(value, flag, epoch):
 
 
 def flag_of(w, epoch):
-    return (w >> 32) & 3 if (w >> 34) == (epoch & EPOCH_MASK) else 0
+    """status_flag: AGG or INCL of this epoch, else 0 (stale or claimed)."""
+    f = (w >> 32) & 3
+    return f if (w >> 34) == (epoch & EPOCH_MASK) and f != CLAIM else 0
+
+
+def claimed(w, epoch):
+    return (w >> 34) == (epoch & EPOCH_MASK) and (w >> 32) & 3 == CLAIM
 
 
 class Memory:
@@ -62,7 +70,7 @@ def stall_then_publish(mem, t, agg, epoch, num_tiles, period, stats):
 def lookback(mem, t, agg, epoch, spin_limit, stats):
     """Generator returning tile t's exclusive prefix (t > 0): window polls,
     fallback reductions and CAS insertions, one yield per access."""
-    prefix, top, spins = 0, t - 1, 0
+    prefix, top, spins, claim_waits = 0, t - 1, 0, 0
     while True:
         # window read: lane l reads tile top - l (one step per word)
         words = []
@@ -87,13 +95,34 @@ def lookback(mem, t, agg, epoch, spin_limit, stats):
         # successor (fallback_patience in scan_u32.hip)
         if spins < spin_limit * (1 + min(t - 1 - top, 7)):
             continue
-        # fallback onto tile `top`: reduce its input, CAS over the stale word
+        # claim_for_fallback: claim tile `top` before reducing it; a tile
+        # already claimed is waited on for up to CLAIM_PATIENCE polls
+        j, w0 = top, words[0]
+        if claimed(w0, epoch):
+            claim_waits += 1
+            if claim_waits < CLAIM_PATIENCE:
+                spins = 0
+                stats["claim_wait"] += 1
+                continue
+            claim_waits, expect = 0, w0
+            stats["claim_forced"] += 1
+        else:
+            claim = make_status(0, CLAIM, epoch)
+            found = mem.cas(j, w0, claim)
+            yield
+            if found != w0:
+                spins = 0
+                stats["claim_lost"] += 1
+                continue
+            claim_waits, expect = 0, claim
+        # fallback onto tile j: reduce its input, CAS over the claim
         stats["fallback"] += 1
-        j, stale = top, words[0]
-        yield  # the redundant reduction of tile j's input
+        for _ in range(REDUCTION_STEPS):  # the redundant reduction of tile j's input
+            yield
         mine = make_status(agg[j], AGG if j else INCL, epoch)
-        found = mem.cas(j, stale, mine)
+        found = mem.cas(j, expect, mine)
         yield
+        stale = expect
         if found == stale:
             stats["cas_win"] += 1
             found, fl = mine, (AGG if j else INCL)
@@ -300,8 +329,33 @@ def test_every_fallback_branch_is_exercised():
         for t, r in enumerate(res):
             assert r == acc
             acc = (acc + agg[t]) & MASK32
-    for branch in ("fallback", "cas_win", "cas_lost_agg", "cas_lost_incl", "fallback_tile0", "stall_released"):
+    for branch in ("fallback", "cas_win", "fallback_tile0", "stall_released", "claim_lost", "claim_wait"):
         assert stats[branch] > 0, (branch, dict(stats))
+    # waiters behind a claimed tile wait for its insertion instead of reducing
+    # it again: every reduction inserts
+    assert stats["fallback"] == stats["cas_win"] + stats["cas_lost_agg"] + stats["cas_lost_incl"]
+    assert stats["fallback"] <= 1.05 * stats["cas_win"]
+
+
+def test_forced_fallback_on_a_long_claim(monkeypatch):
+    # With a tiny claim patience, waiters stop waiting on a claimed tile and
+    # reduce it themselves; the results stay exact.
+    monkeypatch.setitem(globals(), "CLAIM_PATIENCE", 2)
+    monkeypatch.setitem(globals(), "REDUCTION_STEPS", 40)
+    stats = Counter()
+    for seed in range(20):
+        rng = random.Random(3000 + seed)
+        mem = Memory(64)
+        agg = [rng.getrandbits(32) for _ in range(48)]
+        res = run_launch(mem, agg, 1 + seed % 3, period=rng.choice([2, 7]), spin_limit=1,
+                         resident=rng.choice([5, 9]), rng=rng, stats=stats)
+        acc = 0
+        for t, r in enumerate(res):
+            assert r == acc
+            acc = (acc + agg[t]) & MASK32
+    assert stats["claim_forced"] > 0, dict(stats)
+    # a forced reduction's insertion loses to the claimer's (or the owner's) word
+    assert stats["cas_lost_agg"] + stats["cas_lost_incl"] > 0, dict(stats)
 
 
 def test_stale_epoch_words_are_never_consumed():
