@@ -385,26 +385,49 @@ class Legs:
     def tuner(self, out):
         """Learned tuner: regret of the tuner's choices (MLP cost model, inference
         on the GPU with the library's kernels, gated against the default) on the
-        128 held-out workloads of the 512-workload sweep, against the
-        exhaustively measured best; and, without the model, the regret of the
-        library's per-size default, of its fixed 1024x8 default and of the best
-        static rule on the same workloads."""
+        held-out workloads of the 512-workload sweep and of the decoupled-
+        fallback (stall) sweep, against the exhaustively measured best; and,
+        without the model, the regret of the library's per-size default, of its
+        fixed 1024x8 default and of the best static rule on the same workloads.
+        Then 16 held-out workloads (n <= 2^26, <= 2048 segments) are re-timed
+        here, every candidate of each: the drift of those timings against the
+        committed ones, and the regrets on the fresh timings."""
         from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
-        from gpuprefixsums_amd.tuner.evaluate import best_static, load_times, regrets, summarize
+        from gpuprefixsums_amd.tuner.evaluate import best_static, load_sweeps, regrets, summarize
+        from gpuprefixsums_amd.tuner.harness import Runner
         from gpuprefixsums_amd.tuner.model import CostModel
-        from gpuprefixsums_amd.tuner.space import c5_workloads, split
-        times_path = DATA / "c5_times_mi355x.jsonl"
+        times_path, stall_path = DATA / "c5_times_mi355x.jsonl", DATA / "stall_times_mi355x.jsonl"
         if not times_path.exists() or not DEFAULT_MODEL.exists():
             out["tuner"] = None
             return
-        times = load_times(times_path)
-        train, held = split([w for w in c5_workloads() if w.wid in times])
-        static = best_static(train, times)
-        r_mlp, r_def, r_stat, rows = regrets(CostModel.load(DEFAULT_MODEL), held, times, static)
-        out["tuner"] = {"held_out": len(held), "regret_mlp": r_mlp, "regret_default": r_def,
-                        "regret_fixed_default": summarize([r["fixed_regret"] for r in rows]),
-                        "regret_best_static": r_stat, "best_static": static,
-                        "source": "tuner/data/c5_times_mi355x.jsonl (exhaustive harness timings on MI355X)"}
+        times, train, held, s_train, s_held = load_sweeps(times_path, stall_path)
+        static = best_static(train + s_train, times)
+        model = CostModel.load(DEFAULT_MODEL)
+        r_mlp, r_def, r_stat, rows = regrets(model, held, times, static)
+        res = {"held_out": len(held), "regret_mlp": r_mlp, "regret_default": r_def,
+               "regret_fixed_default": summarize([r["fixed_regret"] for r in rows]),
+               "regret_best_static": r_stat, "best_static": static,
+               "source": "tuner/data/c5_times_mi355x.jsonl (exhaustive harness timings on MI355X)"}
+        if s_held:
+            s_mlp, s_def, s_stat, _ = regrets(model, s_held, times, static)
+            res["stall_sweep"] = {"held_out": len(s_held), "regret_mlp": s_mlp, "regret_default": s_def,
+                                  "regret_best_static": s_stat}
+        subset = [w for w in held if w.n <= 1 << 26 and w.segments <= 2048][:16]
+        if subset:
+            runner = Runner(max_n=max(w.n for w in subset), max_segments=max(w.segments for w in subset))
+            fresh, ratios = {}, []
+            for w in subset:
+                fresh[w.wid] = runner.time_all(w)
+                ratios += [fresh[w.wid][c] / times[w.wid][c] for c in fresh[w.wid] if c in times[w.wid]]
+            f_mlp, f_def, _, _ = regrets(model, subset, fresh, static)
+            np = self.np
+            dev = np.abs(np.array(ratios) - 1.0)
+            res["retimed"] = {"workloads": [w.wid for w in subset], "timings": len(ratios),
+                              "drift_median": float(np.median(dev)), "drift_max": float(dev.max()),
+                              "regret_mlp_fresh": f_mlp, "regret_default_fresh": f_def}
+            del runner
+            self.torch.cuda.empty_cache()
+        out["tuner"] = res
 
     def n2p20(self, out):
         gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np

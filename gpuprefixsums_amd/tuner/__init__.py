@@ -1,5 +1,7 @@
-"""Learned tuner: picks the scan configuration (algorithm, tile shape, spin
-limit) for a workload from its size, dtype and segment statistics.
+"""Learned tuner: picks the scan configuration (algorithm, tile shape,
+schedule, lookback spin limit) for a workload from its size, dtype, segment
+statistics and, for the decoupled-fallback scan under emulated deadlock, its
+stall period.
 
     from gpuprefixsums_amd.tuner import Tuner
     cfg = Tuner().choose(workload)          # MLP cost model (trained on the GPU)
@@ -32,7 +34,7 @@ class Tuner:
         self._runner = runner
 
     def predict_ms(self, w: Workload) -> dict:
-        cands = candidates(w.dtype)
+        cands = candidates(w.dtype, w.stall_period)
         X = np.array([features(w, c) for c in cands])
         pred = self.model.predict(X)
         return {c: float(2.0 ** p / 1000.0) for c, p in zip(cands, pred)}
@@ -42,11 +44,12 @@ class Tuner:
 
     def choose(self, w: Workload, fallback: str = "exhaustive") -> Config:
         if self.model is not None and self.in_range(w):
-            cands = candidates(w.dtype)
+            cands = candidates(w.dtype, w.stall_period)
             pred = self.model.predict(np.array([features(w, c) for c in cands]))
-            return gated_choice(cands, pred, default_config(w.dtype, w.segments, w.n), self.model.margin)
+            return gated_choice(cands, pred, default_config(w.dtype, w.segments, w.n, w.stall_period),
+                                self.model.margin)
         if fallback == "default":
-            return default_config(w.dtype, w.segments, w.n)
+            return default_config(w.dtype, w.segments, w.n, w.stall_period)
         if self._runner is None:
             from .harness import Runner
             self._runner = Runner(max_n=max(w.n, 1 << 10))

@@ -2,7 +2,9 @@
 
     python -m gpuprefixsums_amd.tuner.evaluate --times times.jsonl [--save]
 
-Training set: the first 384 workloads of the sweep; held out: the last 128.
+Training set: the first 384 workloads of the C5 sweep and the first 72 of
+the decoupled-fallback (stall) sweep; held out: the last 128 and the last 24.
+One model is trained on both and judged on each held-out set separately.
 For a held-out workload with exhaustively measured times T(c):
     regret(choice) = T(choice) / min_c T(c) - 1
 reported as mean / median / max over the held-out set, for the tuner's
@@ -30,7 +32,7 @@ import numpy as np
 from . import DATA, DEFAULT_MODEL
 from .model import CostModel
 from .space import (Config, Workload, c5_workloads, candidates, default_config, features, fixed_default_config,
-                    gated_choice, split)
+                    gated_choice, split, stall_workloads)
 
 CV_FOLDS = 4
 CV_WEIGHT_DECAY = (1e-5, 1e-4, 1e-3)
@@ -56,15 +58,22 @@ def dataset(ws: List[Workload], times: Dict[int, Dict[str, float]]) -> Tuple[np.
     return np.array(X), np.array(y)
 
 
+def group(w: Workload) -> str:
+    """Key of the model-free static rule: the dtype, or "u32_stall" for the
+    decoupled-fallback sweep."""
+    return "u32_stall" if w.stall_period else w.dtype
+
+
 def best_static(ws: List[Workload], times: Dict[int, Dict[str, float]]) -> Dict[str, str]:
-    """Per dtype, the one configuration with the lowest mean regret over `ws`
+    """Per group, the one configuration with the lowest mean regret over `ws`
     (the strongest model-free rule: a fixed choice tuned on the training set)."""
     out = {}
-    for dt in sorted({w.dtype for w in ws}):
-        sub = [w for w in ws if w.dtype == dt]
-        names = [c.name for c in candidates(dt) if all(c.name in times[w.wid] for w in sub)]
+    for g in sorted({group(w) for w in ws}):
+        sub = [w for w in ws if group(w) == g]
+        names = [c.name for c in candidates(sub[0].dtype, sub[0].stall_period)
+                 if all(c.name in times[w.wid] for w in sub)]
         mean = {c: np.mean([times[w.wid][c] / min(times[w.wid].values()) - 1.0 for w in sub]) for c in names}
-        out[dt] = min(mean, key=mean.get)
+        out[g] = min(mean, key=mean.get)
     return out
 
 
@@ -76,15 +85,15 @@ def regrets(model: CostModel, ws: List[Workload], times: Dict[int, Dict[str, flo
     for w in ws:
         t = times[w.wid]
         best = min(t.values())
-        cands = [c for c in candidates(w.dtype) if c.name in t]
+        cands = [c for c in candidates(w.dtype, w.stall_period) if c.name in t]
         pred = model.predict(np.array([features(w, c) for c in cands]))
-        d = default_config(w.dtype, w.segments, w.n)
+        d = default_config(w.dtype, w.segments, w.n, w.stall_period)
         pick = gated_choice(cands, pred, d, margin)
-        s = static.get(w.dtype, d.name)
+        s = static.get(group(w), d.name)
         mlp.append(t[pick.name] / best - 1.0)
         dflt.append(t[d.name] / best - 1.0)
         stat.append(t[s] / best - 1.0)
-        fx = fixed_default_config(w.dtype, w.segments).name
+        fx = fixed_default_config(w.dtype, w.segments, w.stall_period).name
         rows.append({"wid": w.wid, "n": w.n, "dtype": w.dtype, "segments": w.segments, "best": min(t, key=t.get),
                      "mlp": pick.name, "default": d.name, "static": s, "mlp_regret": mlp[-1],
                      "default_regret": dflt[-1], "static_regret": stat[-1], "fixed_regret": t[fx] / best - 1.0})
@@ -112,9 +121,9 @@ def cross_validate(train: List[Workload], times: Dict[int, Dict[str, float]], se
                 for w in folds[f]:
                     t = times[w.wid]
                     best = min(t.values())
-                    cands = [c for c in candidates(w.dtype) if c.name in t]
+                    cands = [c for c in candidates(w.dtype, w.stall_period) if c.name in t]
                     pred = model.predict(np.array([features(w, c) for c in cands]))
-                    d = default_config(w.dtype, w.segments, w.n)
+                    d = default_config(w.dtype, w.segments, w.n, w.stall_period)
                     for m in CV_MARGINS:
                         per_margin[m].append(t[gated_choice(cands, pred, d, m).name] / best - 1.0)
             for m in CV_MARGINS:
@@ -124,17 +133,28 @@ def cross_validate(train: List[Workload], times: Dict[int, Dict[str, float]], se
             "grid": {f"wd={k[0]:g} epochs={k[1]} margin={k[2]:g}": v for k, v in scores.items()}}
 
 
+def load_sweeps(c5_path: Path, stall_path: Path):
+    """Timings of both sweeps (the stall file is optional) and their
+    train / held-out splits: (times, c5 train, c5 held, stall train, stall held)."""
+    times = load_times(c5_path)
+    if stall_path and Path(stall_path).exists():
+        times.update(load_times(stall_path))
+    train, held = split([w for w in c5_workloads() if w.wid in times])
+    s_train, s_held = split([w for w in stall_workloads() if w.wid in times])
+    return times, train, held, s_train, s_held
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--times", default=str(DATA / "c5_times_mi355x.jsonl"))
+    ap.add_argument("--stall-times", default=str(DATA / "stall_times_mi355x.jsonl"))
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save", action="store_true", help="write the model and the regret table")
     ap.add_argument("--out-dir", default=str(DATA), help="where --save writes (default tuner/data/)")
     args = ap.parse_args(argv)
-    times = load_times(Path(args.times))
-    ws = [w for w in c5_workloads() if w.wid in times]
-    train, held = split(ws)
+    times, c5_train, held, s_train, s_held = load_sweeps(Path(args.times), Path(args.stall_times))
+    train = c5_train + s_train
     if not held:
         print("no held-out workloads measured", file=sys.stderr)
         return 1
@@ -158,12 +178,18 @@ def main(argv=None) -> int:
            "regret_mlp": r_mlp, "regret_default": r_def, "regret_best_static": r_stat,
            "regret_fixed_default": summarize([r["fixed_regret"] for r in rows]),
            "best_static": static}
+    stall_rows = []
+    if s_held:
+        s_mlp, s_def, s_stat, stall_rows = regrets(model, s_held, times, static)
+        res["stall_sweep"] = {"train_workloads": len(s_train), "held_out_workloads": len(s_held),
+                              "regret_mlp": s_mlp, "regret_default": s_def, "regret_best_static": s_stat}
     print(json.dumps(res, indent=1))
     if args.save:
         out = Path(args.out_dir)
         out.mkdir(parents=True, exist_ok=True)
         model.save(out / DEFAULT_MODEL.name)
-        (out / "regret_mi355x.json").write_text(json.dumps({**res, "held_out": rows}, indent=1) + "\n")
+        (out / "regret_mi355x.json").write_text(
+            json.dumps({**res, "held_out": rows, "stall_held_out": stall_rows}, indent=1) + "\n")
     return 0
 
 

@@ -3,7 +3,8 @@ workload on the GPU. Its output is both the cost model's training data and
 the judge of the model's regret; for a workload the model does not cover it
 is the fallback.
 
-    python -m gpuprefixsums_amd.tuner.harness --out times.jsonl [--start 0 --count 512]
+    python -m gpuprefixsums_amd.tuner.harness --out times.jsonl [--sweep c5|stall] [--start 0 --count 512]
+                                              [--configs name,name,...]
 
 One JSON line per workload: {"wid", "n", "dtype", "segments", "times_ms": {config: ms}}.
 Each time is the median of a few back-to-back runs bracketed by events on the
@@ -19,7 +20,7 @@ import time
 from pathlib import Path
 from typing import Dict
 
-from .space import Config, Workload, c5_workloads, candidates
+from .space import Config, Workload, c5_workloads, candidates, stall_workloads
 
 
 class Runner:
@@ -43,13 +44,16 @@ class Runner:
         self.handles: Dict[Config, object] = {}
         torch.cuda.synchronize()
 
-    def handle(self, c: Config):
+    def handle(self, c: Config, stall_period: int = 0):
         gps = self.gps
-        h = self.handles.get(c)
+        key = (c, stall_period)
+        h = self.handles.get(key)
         if h is None:
             algo, kw = c.handle_args()
+            if stall_period:
+                kw["stall_period"] = stall_period
             h = gps.ScanHandle(algo, self.max_n, **kw)
-            self.handles[c] = h
+            self.handles[key] = h
         return h
 
     def stage(self, w: Workload) -> None:
@@ -57,7 +61,7 @@ class Runner:
         self.off[: w.segments + 1].copy_(self.torch.from_numpy(w.offsets.view(np.int64)))
 
     def run(self, w: Workload, c: Config) -> None:
-        h = self.handle(c)
+        h = self.handle(c, w.stall_period)
         if w.dtype == "u32":
             if c.kind == "batched":
                 h.segmented(self.u32, self.out, self.off, w.segments, w.n, False)
@@ -90,11 +94,14 @@ class Runner:
                 break
         return statistics.median(out)
 
-    def time_all(self, w: Workload) -> Dict[str, float]:
+    def time_all(self, w: Workload, only=None) -> Dict[str, float]:
+        """Times every candidate of w (or only those named in `only`)."""
         self.stage(w)
         res: Dict[str, float] = {}
         best = float("inf")
-        for c in candidates(w.dtype):
+        for c in candidates(w.dtype, w.stall_period):
+            if only is not None and c.name not in only:
+                continue
             t = self.time(w, c, best)
             best = min(best, t)
             res[c.name] = t
@@ -107,20 +114,29 @@ class Runner:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--sweep", choices=("c5", "stall"), default="c5")
     ap.add_argument("--start", type=int, default=0)
     ap.add_argument("--count", type=int, default=512)
-    ap.add_argument("--seed", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--configs", default=None, help="comma list: time only these configurations")
     args = ap.parse_args(argv)
-    ws = c5_workloads(args.seed)[args.start:args.start + args.count]
-    runner = Runner()
+    if args.sweep == "c5":
+        ws = c5_workloads(6 if args.seed is None else args.seed)
+    else:
+        ws = stall_workloads(7 if args.seed is None else args.seed)
+    ws = ws[args.start:args.start + args.count]
+    only = set(args.configs.split(",")) if args.configs else None
+    if only is not None:
+        ws = [w for w in ws if any(c.name in only for c in candidates(w.dtype, w.stall_period))]
+    runner = Runner(max_n=max(w.n for w in ws) if ws else 1 << 10)
     path = Path(args.out)
     path.parent.mkdir(parents=True, exist_ok=True)
     t0 = time.time()
     with path.open("a") as f:
         for w in ws:
-            res = runner.time_all(w)
+            res = runner.time_all(w, only)
             f.write(json.dumps({"wid": w.wid, "n": w.n, "dtype": w.dtype, "segments": w.segments,
-                                "times_ms": res}) + "\n")
+                                "stall_period": w.stall_period, "times_ms": res}) + "\n")
             f.flush()
             print(f"[harness] workload {w.wid} n={w.n} {w.dtype} S={w.segments} best "
                   f"{min(res, key=res.get)} {min(res.values()):.4f} ms ({time.time() - t0:.0f} s)",

@@ -62,25 +62,29 @@ def test_training_reduces_loss_and_fits():
 
 def test_trained_tuner_regret_on_held_out_workloads():
     """The shipped model's held-out regret against exhaustive search, and what
-    it gains over the methods without the model: the library's fixed default
-    and the best static rule. The library's per-size default, a rule fitted by
-    hand to the same training sweep, is matched within one point."""
+    it gains over the methods without the model: the library's per-size
+    default, its fixed default and the best static rule, on the 128 held-out
+    workloads of the C5 sweep and the 24 of the decoupled-fallback (stall)
+    sweep, where the choice includes the lookback spin limit."""
     from gpuprefixsums_amd.tuner import DATA, DEFAULT_MODEL
-    from gpuprefixsums_amd.tuner.evaluate import best_static, load_times, regrets, summarize
-    from gpuprefixsums_amd.tuner.space import c5_workloads, split
-    times_path = DATA / "c5_times_mi355x.jsonl"
-    if not times_path.exists() or not DEFAULT_MODEL.exists():
+    from gpuprefixsums_amd.tuner.evaluate import best_static, load_sweeps, regrets, summarize
+    times_path, stall_path = DATA / "c5_times_mi355x.jsonl", DATA / "stall_times_mi355x.jsonl"
+    if not times_path.exists() or not stall_path.exists() or not DEFAULT_MODEL.exists():
         pytest.skip("tuner data not generated yet")
-    times = load_times(times_path)
-    ws = [w for w in c5_workloads() if w.wid in times]
-    train, held = split(ws)
-    r_mlp, r_def, r_stat, rows = regrets(CostModel.load(DEFAULT_MODEL), held, times, best_static(train, times))
+    times, train, held, s_train, s_held = load_sweeps(times_path, stall_path)
+    assert len(held) == 128 and len(s_held) == 24
+    model = CostModel.load(DEFAULT_MODEL)
+    static = best_static(train + s_train, times)
+    r_mlp, r_def, r_stat, rows = regrets(model, held, times, static)
     r_fixed = summarize([r["fixed_regret"] for r in rows])
-    print(json.dumps({"mlp": r_mlp, "per_size_default": r_def, "fixed_default": r_fixed, "best_static": r_stat}))
+    s_mlp, s_def, s_stat, _ = regrets(model, s_held, times, static)
+    print(json.dumps({"mlp": r_mlp, "per_size_default": r_def, "fixed_default": r_fixed, "best_static": r_stat,
+                      "stall_mlp": s_mlp, "stall_default": s_def, "stall_static": s_stat}))
     assert r_mlp["mean"] <= 0.05
-    assert r_mlp["mean"] < r_fixed["mean"]
-    assert r_mlp["mean"] < r_stat["mean"]
-    assert r_mlp["mean"] <= r_def["mean"] + 0.01
+    assert r_mlp["mean"] < r_def["mean"]
+    assert r_mlp["mean"] < r_fixed["mean"] and r_mlp["mean"] < r_stat["mean"]
+    assert s_mlp["mean"] <= 0.05
+    assert s_mlp["mean"] < s_def["mean"]
 
 
 def test_tuner_choose_with_the_model_and_by_exhaustive_search():

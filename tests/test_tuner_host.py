@@ -7,8 +7,9 @@ import numpy as np
 import pytest
 
 from gpuprefixsums_amd.tuner.evaluate import best_static, regrets
-from gpuprefixsums_amd.tuner.space import (FEATURES, KINDS, Config, Workload, c5_workloads, candidates, default_config,
-                                           features, gated_choice, lognormal_partition, split)
+from gpuprefixsums_amd.tuner.space import (FEATURES, KINDS, STALL_CONFIGS, Config, Workload, c5_workloads, candidates,
+                                           default_config, features, gated_choice, lognormal_partition, split,
+                                           stall_workloads)
 
 
 def test_sweep_shape_and_split():
@@ -49,6 +50,9 @@ def test_config_names_round_trip_and_candidates():
     # the library's size-adaptive defaults
     assert [default_config("u32", 9, n).name for n in (1 << 19, 1 << 21, 1 << 22)] == [
         "batched:1024x2", "batched:1024x4", "batched:1024x8"]
+    assert [default_config("u32", 1, n).name for n in (1 << 18, (1 << 18) + 1, 1 << 21, (1 << 24) - 1, 1 << 24)] == [
+        "loop_csdl:1024x2:tile", "loop_csdl:1024x4:tile", "loop_csdl:1024x4:tile", "loop_csdl:1024x8:tile",
+        "loop_csdl:1024x8"]
     assert [default_config("f32", 1, n).name for n in (1 << 18, 1 << 22, (1 << 22) + 1, 1 << 24, (1 << 24) + 1)] == [
         "loop_fp:256x4", "loop_fp:1024x4", "loop_fp:1024x8", "loop_fp:1024x8", "loop_fp:1024x8:pp"]
     assert [default_config("bf16", 9, n).name for n in ((1 << 20) - 1, 1 << 20, 1 << 23)] == [
@@ -61,13 +65,35 @@ def test_config_names_round_trip_and_candidates():
 def test_features_dimension_and_values():
     w = Workload(0, 1 << 20, "bf16", 4, np.array([0, 1, 2, 3, 1 << 20], dtype=np.uint64))
     f = features(w, Config("loop_csdldf", 1024, 4, 32))
-    assert len(f) == len(FEATURES) == 7 + len(KINDS) + 4
+    assert len(f) == len(FEATURES) == 8 + len(KINDS) + 4
     d = dict(zip(FEATURES, f))
     assert d["log2_n"] == 20 and d["log2_segments"] == 2
-    assert d["is_bf16"] == 1 and d["is_u32"] == 0
+    assert d["is_bf16"] == 1 and d["is_u32"] == 0 and d["log2_stall"] == 0
     assert d["kind_loop_csdldf"] == 1 and d["log2_block"] == 10 and d["log2_spin"] == math.log2(33)
     assert d["persistent"] == 0 and dict(zip(FEATURES, features(w, Config("loop_fp", 1024, 8, persistent=True))))[
         "persistent"] == 1
+    # the u32 kernels run persistent unless marked ":tile"
+    assert dict(zip(FEATURES, features(w, Config("loop_csdl", 1024, 8))))["persistent"] == 1
+    assert dict(zip(FEATURES, features(w, Config("loop_csdl", 1024, 8, tile=True))))["persistent"] == 0
+    ws = stall_workloads()[0]
+    assert dict(zip(FEATURES, features(ws, STALL_CONFIGS[0])))["log2_stall"] == math.log2(ws.stall_period + 1)
+
+
+def test_stall_sweep_and_its_candidates():
+    ws = stall_workloads()
+    assert len(ws) == 96 and ws[0].wid == 10000
+    assert {w.stall_period for w in ws} == {7, 8}
+    assert all(2 ** 16 - 1 <= w.n <= 2 ** 26 + 1 and w.segments == 1 and w.dtype == "u32" for w in ws)
+    train, held = split(ws)
+    assert len(train) == 72 and len(held) == 24
+    cs = candidates("u32", 7)
+    assert cs == STALL_CONFIGS and {c.kind for c in cs} == {"loop_csdldf"}
+    assert {c.spin for c in cs} == {1, 2, 4, 8, 16, 32}
+    assert default_config("u32", 1, 1 << 20, stall_period=8) == Config("loop_csdldf", 1024, 8, 4)
+    assert default_config("u32", 1, 1 << 20, stall_period=8) in cs
+    import gpuprefixsums_amd as gps
+    assert Config("loop_csdl", 1024, 2, tile=True).handle_args() == (
+        gps.ALGO_CSDL, dict(block_threads=1024, vec_per_thread=2, schedule=gps.SCHEDULE_TILE))
 
 
 class _StubModel:
@@ -85,7 +111,8 @@ class _StubModel:
             kind = [k for k in KINDS if row[col[f"kind_{k}"]] == 1][0]
             c = Config(kind, int(round(2 ** row[col["log2_block"]])), int(round(2 ** row[col["log2_vec"]])),
                        int(round(2 ** row[col["log2_spin"]] - 1)) if kind == "loop_csdldf" else 0,
-                       bool(row[col["persistent"]]))
+                       bool(row[col["persistent"]]) and kind in ("loop_fp", "batched_fp"),
+                       kind == "loop_csdl" and not row[col["persistent"]])
             out.append(self.table.get(c.name, 100.0))
         return np.array(out)
 
