@@ -115,3 +115,14 @@ def test_survey_segment_lengths():
     assert {k: gps.survey_segment(v) for k, v in gps.SURVEY_NETWORKS.items()} == want
     with pytest.raises(gps.GpsError):
         gps.survey_segment(len(want))
+
+
+def test_sized_buffers_are_checked_against_n():
+    # An explicit n is checked against tensors' sizes before any call.
+    torch = pytest.importorskip("torch")
+    import gpuprefixsums_amd as gps
+    t = torch.zeros(100, dtype=torch.int32)
+    gps._need("scan", 400, t)  # exactly enough
+    with pytest.raises(ValueError, match="400 bytes"):
+        gps._need("scan", 404, t)
+    gps._need("scan", 1 << 40, 12345)  # raw addresses are not sized

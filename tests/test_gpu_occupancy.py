@@ -70,7 +70,10 @@ def test_occupancy_argument_errors():
     with gps.ScanHandle(gps.ALGO_CSDL, 4096) as h:
         with pytest.raises(gps.GpsError, match="positive"):
             h.estimate_occupancy(d, d, 0)
+        big = torch.zeros(8192, dtype=torch.int32, device="cuda")
         with pytest.raises(gps.GpsError, match="max_n"):
+            h.estimate_occupancy(big, big, 4097)
+        with pytest.raises(ValueError, match="buffer"):  # n past the tensors: refused on the host
             h.estimate_occupancy(d, d, 4097)
         with pytest.raises(gps.GpsError, match="16-byte"):
             h.estimate_occupancy(d.data_ptr() + 4, d, 100, struct_scan=True)

@@ -129,7 +129,10 @@ def test_segmented_argument_errors():
         with pytest.raises(gps.GpsError):
             h.segmented(d, d, off, 1, 1000)
     with gps.ScanHandle(gps.ALGO_CSDL, 1000) as h:
-        with pytest.raises(gps.GpsError):
+        big = sentinel_u32(1001)
+        with pytest.raises(gps.GpsError):  # n above the handle's max_n
+            h.segmented(big, big, off, 1, 1001)
+        with pytest.raises(ValueError, match="buffer"):  # n past the tensors: refused on the host
             h.segmented(d, d, off, 1, 1001)
         with pytest.raises(gps.GpsError):
             h.segmented(d, d, 0, 1, 1000)
