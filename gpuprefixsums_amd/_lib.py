@@ -93,6 +93,7 @@ SIGNATURES = {
     "gps_device_error": (_int, [_vp, ctypes.POINTER(_u32)]),
     "gps_reset": (_int, [_vp, _vp]),
     "gps_time_scan_u32": (_int, [_vp, _vp, _vp, _u64, _int, _vp, _u32, ctypes.POINTER(ctypes.c_float)]),
+    "gps_scan_u32_noalloc": (_int, [_vp, _vp, _u64, _int, _vp]),
     "gps_fill_u32": (_int, [_vp, _u64, _u32, _vp]),
     "gps_copy_u32": (_int, [_vp, _vp, _u64, _vp]),
     "gps_fill_random_u32": (_int, [_vp, _u64, _u64, _u64, _u32, _vp]),

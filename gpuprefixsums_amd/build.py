@@ -40,6 +40,7 @@ LIB_SOURCES = [
     "capi.cpp",
     "capi_ext.cpp",
     "dispatchers.cpp",
+    "noalloc.hip",
 ]
 
 # The comparator library: rocPRIM's device scan and the CubDispatcher class.

@@ -276,6 +276,19 @@ int gps_device_error(gps_handle* h, uint32_t* bits);
  * points do this themselves, graph replays must call it). */
 int gps_reset(gps_handle* h, void* stream);
 
+/* Allocation-free uint32 scan (no counterpart in the reference). For callers
+ * that cannot allocate device memory: code under stream capture, serving
+ * loops, harnesses that forbid hipMalloc. Same result as gps_scan_u32 on a
+ * default GPS_ALGO_CSDL handle, but with no handle: the scratch (status slots,
+ * ticket and error words) is a static device array of the library, 1 MiB +
+ * 256 B per GPU, held until the process ends; the call allocates nothing and
+ * launches one kernel (two on the first call on a device, which also re-arms
+ * the scratch). n at most 2^28 (GPS_ERR_TOO_LARGE beyond). Every call on a
+ * GPU shares that GPU's scratch: calls that overlap in time on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller.
+ * Device watchdog bits are not reported (no handle to read them from). */
+int gps_scan_u32_noalloc(const uint32_t* d_in, uint32_t* d_out, uint64_t n, int inclusive, void* stream);
+
 /* Times `iters` back-to-back calls of the handle's scan on `stream` with a
  * hipEvent pair around each call; writes per-call milliseconds to ms[iters].
  * Cf. BatchTimingInclusive (...Dispatcher.cuh:116-156). */
