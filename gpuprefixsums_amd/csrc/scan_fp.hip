@@ -1009,6 +1009,185 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent segmented FP scan over lane-blocked runs (pair carries; the
+// persistent schedule's kernel for them). segscan_fp_pp's tile loop and head
+// discovery with scan_fp_bl's per-wave LDS transpose: lane l of wave w holds
+// tile elements w*256*VEC + 4*VEC*l .. + 4*VEC - 1, whose head bits are
+// 4*VEC consecutive bits of the tile's head bitmap (one word at VEC = 8). The
+// lane scans its run serially with the carry reset at heads, one segmented
+// pair wave scan per run gives every lane its carry-in, and the outputs are
+// recomputed serially and transposed back.
+// ---------------------------------------------------------------------------
+template <int VEC>
+__device__ __forceinline__ uint32_t lane_run_heads(const uint32_t* s_heads, uint32_t run_base, uint32_t lane) {
+    constexpr uint32_t E = 4u * VEC;  // elements per lane run
+    const uint32_t first = run_base + E * lane;
+    const uint32_t word = s_heads[first / 32u];
+    if (E == 32u) return word;
+    return (word >> (first % 32u)) & ((1u << E) - 1u);
+}
+
+template <int BLOCK, int VEC, bool BF16, bool COMP>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void segscan_fp_bl(SegFpParams p) {
+    using C = TileCfg<BLOCK, VEC>;
+    using Ops = PairCarry<COMP>;
+    constexpr uint32_t kWords = C::kTile / 32;
+    constexpr uint32_t kSearchWave = C::kWaves > 1 ? C::kWaves - 1 : 0;
+    __shared__ float s_slices[C::kWaves * BlockedSlice<VEC>::kFloats];
+    __shared__ uint32_t s_heads[2][kWords];
+    __shared__ float s_ws[C::kWaves], s_wc[C::kWaves];
+    __shared__ uint32_t s_wf[C::kWaves];
+    __shared__ uint32_t s_ticket[3];
+    __shared__ float s_ps, s_pc;
+    __shared__ uint64_t s_first_seg;
+    const uint32_t wave = wave_index();
+    const uint32_t lane = lane_id();
+    const uint32_t total_tickets = p.num_tiles + gridDim.x;
+    float* slice = s_slices + wave * BlockedSlice<VEC>::kFloats;
+    if (threadIdx.x == 0) {
+        const Ticket t = take_ticket(p.ticket, total_tickets);
+        s_ticket[0] = t.tile;
+        s_ticket[1] = t.epoch;
+    }
+    for (uint32_t i = threadIdx.x; i < kWords; i += BLOCK) s_heads[0][i] = 0u;
+    __syncthreads();
+    uint32_t tile = s_ticket[0];
+    const uint32_t epoch = s_ticket[1];
+    if (tile >= p.num_tiles) return;
+
+    f32x4 x[VEC], y[VEC];
+    load_fp<BF16, VEC>(x, p.in, thread_base<BLOCK, VEC>(tile), p.n, p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n));
+    {
+        const uint64_t lo = (uint64_t)tile * C::kTile;
+        const uint64_t hi = lo + C::kTile < p.n ? lo + C::kTile : p.n;
+        if (wave == 0) {
+            const uint64_t s0 = find_first_segment(p.offsets, p.num_segments, lo);
+            if (lane == 0) s_first_seg = s0;
+        }
+        __syncthreads();
+        mark_heads<BLOCK>(s_heads[0], p.offsets, p.num_segments, s_first_seg, lo, hi, tile == 0);
+        __syncthreads();
+    }
+
+    const uint32_t run_base = wave * C::kWaveElems;
+    uint32_t cur = 0;
+    while (true) {
+        // Lane run: segmented serial sum (carry reset at heads), then one
+        // segmented pair wave scan for the lane's carry-in.
+        striped_to_blocked<VEC>(x, slice);
+        const uint32_t heads = lane_run_heads<VEC>(s_heads[cur], run_base, lane);
+        ff t = {0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const ff a = ((heads >> (4 * q + c)) & 1u) ? ff{0.f, 0.f} : t;
+                t = Ops::add1(a, x[q][c]);
+            }
+        }
+        uint32_t sf = heads != 0u;
+        ff sv = t;
+        wave_seg_scan_ff<COMP>(sf, sv);
+        const uint32_t ef = GPS_DPP_I(sf, 0x138, 0xf, true);  // wave_shr:1, lane 0 gets 0
+        const ff ev = Ops::shr1(sv);
+        if (lane == 0) {
+            s_ws[wave] = readlane_f(sv.s, 63);
+            s_wc[wave] = COMP ? readlane_f(sv.c, 63) : 0.f;
+            s_wf[wave] = readlane(sf, 63);
+        }
+        __syncthreads();
+        uint32_t pre_f, tile_f;
+        ff pre, tile_tot;
+        {
+            const uint32_t q = lane < C::kWaves ? lane : 0u;
+            uint32_t f = lane < C::kWaves ? s_wf[q] : 0u;
+            ff v = {lane < C::kWaves ? s_ws[q] : 0.f, lane < C::kWaves ? s_wc[q] : 0.f};
+            wave_seg_scan_ff<COMP>(f, v);
+            const uint32_t wef = GPS_DPP_I(f, 0x138, 0xf, true);
+            pre_f = readlane(wef, (int)wave);
+            pre = Ops::lane(Ops::shr1(v), (int)wave);
+            tile_f = readlane(f, C::kWaves - 1);
+            tile_tot = Ops::lane(v, C::kWaves - 1);
+        }
+        const bool need_lookback = tile != 0 && (s_heads[cur][0] & 1u) == 0u;
+        if (threadIdx.x == 0) {
+            // a tile with a head knows its carry-out: publish it inclusive now
+            publish_ff<COMP>(p.status, tile, tile_tot, (tile_f || tile == 0) ? kFlagIncl : kFlagAgg, epoch);
+            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+        }
+        // the other head buffer was last read by the previous tile's output pass
+        for (uint32_t i = threadIdx.x; i < kWords; i += BLOCK) s_heads[cur ^ 1u][i] = 0u;
+        __syncthreads();
+        const uint32_t next = s_ticket[2];
+        const bool has_next = next < p.num_tiles;
+
+        LookbackW2<Ops> lb;
+        if (wave == 0 && need_lookback) {
+            lb.start(tile);
+            lb.issue(p.status, epoch);
+        }
+        if (has_next && wave == kSearchWave) {
+            const uint64_t lo = (uint64_t)next * C::kTile;
+            const uint64_t hi = lo + C::kTile < p.n ? lo + C::kTile : p.n;
+            const uint64_t s0 = find_first_segment(p.offsets, p.num_segments, lo);
+            mark_heads_by(s_heads[cur ^ 1u], p.offsets, p.num_segments, s0, lo, hi, next == 0, lane, kWave);
+        }
+        if (has_next)
+            load_fp<BF16, VEC>(y, p.in, thread_base<BLOCK, VEC>(next), p.n,
+                               p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n));
+        if (wave == 0) {
+            ff prefix = {0.f, 0.f};
+            if (need_lookback) {
+                while (!lb.consume(epoch, p.err)) lb.issue(p.status, epoch);
+                prefix = lb.prefix;
+                if (!tile_f && lane == 0)
+                    publish_ff<COMP>(p.status, tile, ff_add<COMP>(prefix, tile_tot), kFlagIncl, epoch);
+            }
+            if (lane == 0) {
+                s_ps = prefix.s;
+                s_pc = prefix.c;
+            }
+        }
+        __syncthreads();  // also completes the next tile's head bitmap
+        // The lane's carry-in: its segmented exclusive value, plus the run's
+        // carry-in when no head precedes the lane in its wave.
+        const ff run_in = pre_f ? pre : ff_add<COMP>({s_ps, s_pc}, pre);
+        ff acc = ef ? ev : ff_add<COMP>(run_in, ev);
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            f32x4 o;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((heads >> (4 * q + c)) & 1u) acc = {0.f, 0.f};
+                acc = Ops::add1(acc, x[q][c]);
+                o[c] = Ops::round(acc);
+            }
+            x[q] = o;
+        }
+        blocked_to_striped<VEC>(x, slice);
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n)) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+                __builtin_nontemporal_store(x[k], reinterpret_cast<f32x4*>(p.out + base + (uint64_t)k * 256u));
+        } else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint64_t i = base + (uint64_t)k * 256u + (uint64_t)c;
+                    if (i < p.n) p.out[i] = x[k][c];
+                }
+        }
+        if (!has_next) break;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) x[k] = y[k];
+        tile = next;
+        cur ^= 1u;
+    }
+}
+
 #define GPS_FP_SHAPES(X) \
     X(256, 4)            \
     X(512, 8)            \
@@ -1084,10 +1263,18 @@ hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, co
 
 template <int B, int V, bool BF16, bool COMP>
 static hipError_t launch_segfp_pp(const SegFpParams& p, hipStream_t stream) {
-    const uint32_t resident = resident_workgroups<segscan_fp_pp<B, V, BF16, COMP>, B>();
-    if (resident == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
-    hipLaunchKernelGGL((segscan_fp_pp<B, V, BF16, COMP>), dim3(grid), dim3(B), 0, stream, p);
+    // Pair carries run the lane-blocked kernel; plain carries the striped one.
+    if constexpr (COMP) {
+        const uint32_t resident = resident_workgroups<segscan_fp_bl<B, V, BF16, COMP>, B>();
+        if (resident == 0) return hipErrorInvalidDevice;
+        const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+        hipLaunchKernelGGL((segscan_fp_bl<B, V, BF16, COMP>), dim3(grid), dim3(B), 0, stream, p);
+    } else {
+        const uint32_t resident = resident_workgroups<segscan_fp_pp<B, V, BF16, COMP>, B>();
+        if (resident == 0) return hipErrorInvalidDevice;
+        const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+        hipLaunchKernelGGL((segscan_fp_pp<B, V, BF16, COMP>), dim3(grid), dim3(B), 0, stream, p);
+    }
     return hipGetLastError();
 }
 

@@ -130,21 +130,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(GPS_WAVES
 // lowest tile without a published aggregate belongs to a workgroup whose
 // current tile is lower still and already has every predecessor published.
 // ---------------------------------------------------------------------------
-// Tile layout of the persistent scan: the wave-run layout of the other
-// kernels, or, with GPS_PP_SLAB=1, slabs (tile_u32.hpp). At 1024 x 8 and
-// 2^28 elements slabs measured 6036 against 6123 GB/s; a ticketed persistent
-// copy of the scan's shape gains from slabs (6134 against 6016), so the
-// lookback and scan, not the access pattern, decide it for the scan.
-#ifndef GPS_PP_SLAB
-#define GPS_PP_SLAB 0
-#endif
-
 template <int BLOCK, int VEC, bool INCL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(VEC >= 8 ? 4 : GPS_WAVES_PER_EU))) void
 csdl_pp_u32(ScanParams p) {
     using C = TileCfg<BLOCK, VEC>;
-    constexpr bool kSlab = GPS_PP_SLAB != 0;
-    __shared__ uint32_t s_wave[kSlab ? C::kWaves * VEC : C::kWaves];
+    __shared__ uint32_t s_wave[C::kWaves];
     __shared__ uint32_t s_ticket[3];
     __shared__ uint32_t s_prefix;
     __shared__ uint32_t s_carry;
@@ -163,50 +153,19 @@ csdl_pp_u32(ScanParams p) {
     if (tile >= p.num_tiles) return;
     if (threadIdx.x == 0) atomic_add_counter(p.occupancy, 1ull);
 
-    // Loads tile t into d (full tiles with dwordx4 nontemporal loads).
-    auto load_tile = [&](u32x4(&d)[VEC], uint32_t t) {
-        const bool full = p.vec_ok && tile_is_full<BLOCK, VEC>(t, p.n);
-        if constexpr (kSlab) {
-            const uint64_t base = slab_thread_base<BLOCK, VEC>(t);
-            if (full)
-                load_slab<BLOCK, VEC>(d, p.in + base);
-            else
-                load_slab_guarded<BLOCK, VEC>(d, p.in, base, p.n);
-        } else {
-            const uint64_t base = thread_base<BLOCK, VEC>(t);
-            if (full)
-                load_vec<VEC>(d, p.in + base);
-            else
-                load_guarded<VEC>(d, p.in, base, p.n);
-        }
-    };
-
     u32x4 a[VEC], b[VEC];
-    load_tile(a, tile);
+    const uint64_t base = thread_base<BLOCK, VEC>(tile);
+    if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
+        load_vec<VEC>(a, p.in + base);
+    else
+        load_guarded<VEC>(a, p.in, base, p.n);
 
     while (true) {
-        // Scan the tile in registers; part_prefix[k] is the exclusive prefix
-        // inside the tile of this wave's vector k (the same for every k in
-        // the wave-run layout).
-        uint32_t part_prefix[VEC], tile_total;
-        if constexpr (kSlab) {
-            uint32_t tot[VEC];
-            slab_scan_parts<VEC, INCL>(a, tot);
-            if (lane == 0) {
-#pragma unroll
-                for (int k = 0; k < VEC; ++k) s_wave[wave * VEC + k] = tot[k];
-            }
-            __syncthreads();
-            combine_slabs<C::kWaves, VEC>(s_wave, wave, part_prefix, tile_total);
-        } else {
-            const uint32_t run = wave_scan_run<VEC, INCL>(a);
-            if (lane == 0) s_wave[wave] = run;
-            __syncthreads();
-            uint32_t wave_prefix;
-            combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) part_prefix[k] = wave_prefix;
-        }
+        const uint32_t run = wave_scan_run<VEC, INCL>(a);
+        if (lane == 0) s_wave[wave] = run;
+        __syncthreads();
+        uint32_t wave_prefix, tile_total;
+        combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
         if (threadIdx.x == 0) {
             // Publish first, then take the next ticket: ticket order then
             // follows publication order, so lookbacks find contiguous ready
@@ -229,7 +188,13 @@ csdl_pp_u32(ScanParams p) {
             lb.start(tile);
             lb.issue(p.status, epoch);
         }
-        if (has_next) load_tile(b, next);
+        if (has_next) {
+            const uint64_t next_base = thread_base<BLOCK, VEC>(next);
+            if (p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n))
+                load_vec<VEC>(b, p.in + next_base);
+            else
+                load_guarded<VEC>(b, p.in, next_base, p.n);
+        }
         if (wave == 0) {
             uint32_t prefix = tile == 0 ? s_carry : 0u;  // tile 0 starts from the carry-in
             if (tile != 0) {
@@ -243,24 +208,12 @@ csdl_pp_u32(ScanParams p) {
             }
         }
         __syncthreads();
-        const bool full = p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n);
-        if constexpr (kSlab) {
-            uint32_t add[VEC];
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) add[k] = s_prefix + part_prefix[k];
-            const uint64_t base = slab_thread_base<BLOCK, VEC>(tile);
-            if (full)
-                store_slab<BLOCK, VEC>(p.out + base, a, add);
-            else
-                store_slab_guarded<BLOCK, VEC>(p.out, a, add, base, p.n);
-        } else {
-            const uint32_t add = s_prefix + part_prefix[0];
-            const uint64_t base = thread_base<BLOCK, VEC>(tile);
-            if (full)
-                store_vec<VEC>(p.out + base, a, add);
-            else
-                store_guarded<VEC>(p.out, a, add, base, p.n);
-        }
+        const uint32_t add = s_prefix + wave_prefix;
+        const uint64_t base = thread_base<BLOCK, VEC>(tile);
+        if (p.vec_ok && tile_is_full<BLOCK, VEC>(tile, p.n))
+            store_vec<VEC>(p.out + base, a, add);
+        else
+            store_guarded<VEC>(p.out, a, add, base, p.n);
         if (!has_next) break;
 #pragma unroll
         for (int k = 0; k < VEC; ++k) a[k] = b[k];
@@ -1095,7 +1048,9 @@ hipError_t launch_csdl_u32(TileShape s, bool inclusive, const ScanParams& p, hip
 }
 
 // Persistent grid: as many workgroups as fit on the device at once, at most
-// GPS_PP_WG_PER_CU per CU.
+// one per CU. A second resident workgroup per CU measured slower for every
+// shape that allows one at 2^28 (1024 x 4: 5068 against 5396 GB/s; 512 x 8:
+// 5084 against 5474); 1024 x 8 fits one per CU anyway.
 #ifndef GPS_PP_WG_PER_CU
 #define GPS_PP_WG_PER_CU 1
 #endif

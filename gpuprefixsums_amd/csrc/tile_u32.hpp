@@ -54,20 +54,11 @@ __device__ __forceinline__ void load_guarded(u32x4 (&d)[VEC], const uint32_t* in
     }
 }
 
-#ifndef GPS_STORE_NT
-#define GPS_STORE_NT 1  // full-tile stores with the nontemporal hint (streamed once)
-#endif
-
 template <int VEC>
 __device__ __forceinline__ void store_vec(uint32_t* p, const u32x4 (&d)[VEC], uint32_t add) {
     u32x4* v = reinterpret_cast<u32x4*>(p);
 #pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-        if (GPS_STORE_NT)
-            __builtin_nontemporal_store(d[k] + add, v + k * 64);
-        else
-            v[k * 64] = d[k] + add;
-    }
+    for (int k = 0; k < VEC; ++k) __builtin_nontemporal_store(d[k] + add, v + k * 64);
 }
 
 template <int VEC>
@@ -136,116 +127,6 @@ __device__ __forceinline__ void combine_waves(const uint32_t* s_wave, uint32_t w
     const uint32_t s = wave_incl_scan(v);
     wave_prefix = __builtin_amdgcn_readlane(s - v, wave);
     tile_total = readlane(s, WAVES - 1);
-}
-
-// ---------------------------------------------------------------------------
-// Slab layout (the persistent scan's layout). Vector k of the workgroup is
-// one slab of BLOCK*4 contiguous elements; wave w's part of slab k starts at
-// k*BLOCK*4 + w*256 and lane l's uint4 at + 4l:
-//   element of (vector k, wave w, lane l, component c) =
-//       tile * TILE + k * (BLOCK * 4) + w * 256 + l * 4 + c
-// Each load or store instruction of the workgroup then covers one
-// contiguous BLOCK*16-byte slab, and the workgroup walks its tile slab by
-// slab. (With the wave-run layout the waves' concurrent 1 KiB requests sit
-// 256*VEC*4 bytes apart, a power-of-two stride; a 1024 x 8 copy measured
-// 5866 GB/s with that layout against 6281 GB/s with slabs.)
-// ---------------------------------------------------------------------------
-template <int BLOCK, int VEC>
-__device__ __forceinline__ uint64_t slab_thread_base(uint32_t tile) {
-    return (uint64_t)tile * TileCfg<BLOCK, VEC>::kTile + (threadIdx.x / kWave) * 256u + lane_id() * 4u;
-}
-
-template <int BLOCK, int VEC>
-__device__ __forceinline__ void load_slab(u32x4 (&d)[VEC], const uint32_t* p) {
-    const u32x4* v = reinterpret_cast<const u32x4*>(p);
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) d[k] = __builtin_nontemporal_load(v + k * BLOCK);
-}
-
-template <int BLOCK, int VEC>
-__device__ __forceinline__ void load_slab_guarded(u32x4 (&d)[VEC], const uint32_t* in, uint64_t base, uint64_t n) {
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const uint64_t i = base + (uint64_t)k * (BLOCK * 4u) + (uint64_t)c;
-            d[k][c] = i < n ? in[i] : 0u;
-        }
-    }
-}
-
-template <int BLOCK, int VEC>
-__device__ __forceinline__ void store_slab(uint32_t* p, const u32x4 (&d)[VEC], const uint32_t (&add)[VEC]) {
-    u32x4* v = reinterpret_cast<u32x4*>(p);
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) __builtin_nontemporal_store(d[k] + add[k], v + k * BLOCK);
-}
-
-template <int BLOCK, int VEC>
-__device__ __forceinline__ void store_slab_guarded(uint32_t* out, const u32x4 (&d)[VEC], const uint32_t (&add)[VEC],
-                                                   uint64_t base, uint64_t n) {
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const uint64_t i = base + (uint64_t)k * (BLOCK * 4u) + (uint64_t)c;
-            if (i < n) out[i] = d[k][c] + add[k];
-        }
-    }
-}
-
-// Scans each of the wave's VEC slab parts (256 elements each) on its own:
-// on return d holds the inclusive (INCL) or exclusive scan of its part and
-// total[k] (uniform) the part's sum.
-template <int VEC, bool INCL>
-__device__ __forceinline__ void slab_scan_parts(u32x4 (&d)[VEC], uint32_t (&total)[VEC]) {
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-        u32x4 v = d[k];
-        v.y += v.x;
-        v.z += v.y;
-        v.w += v.z;
-        const uint32_t t = v.w;
-        const uint32_t s = wave_incl_scan(t);
-        const uint32_t ex = s - t;
-        if (INCL) {
-            d[k] = v + ex;
-        } else {
-            d[k] = (u32x4){ex, v.x + ex, v.y + ex, v.z + ex};
-        }
-        total[k] = readlane(s, 63);
-    }
-}
-
-// Block combine of the slab parts. s_tot holds WAVES*VEC part totals, wave
-// w's VEC totals at s_tot[w*VEC .. w*VEC+VEC-1] (after a barrier). In tile
-// order part (k, w) is entry k*WAVES + w. One wave scan over those entries
-// (two per lane at most) gives every wave the exclusive prefix of each of
-// its parts and the tile total.
-template <int WAVES, int VEC>
-__device__ __forceinline__ void combine_slabs(const uint32_t* s_tot, uint32_t wave, uint32_t (&prefix)[VEC],
-                                              uint32_t& tile_total) {
-    constexpr uint32_t E = (uint32_t)(WAVES * VEC);
-    static_assert(E <= 128, "at most two entries per lane");
-    constexpr uint32_t R = E > 64 ? 2u : 1u;
-    const uint32_t lane = lane_id();
-    uint32_t e[R];
-#pragma unroll
-    for (uint32_t r = 0; r < R; ++r) {
-        const uint32_t i = lane * R + r;  // entry in tile order: part (i / WAVES, i % WAVES)
-        e[r] = i < E ? s_tot[(i % WAVES) * VEC + i / WAVES] : 0u;
-    }
-    const uint32_t t = R == 2 ? e[0] + e[R - 1] : e[0];
-    const uint32_t s = wave_incl_scan(t);
-    const uint32_t ex = s - t;
-#pragma unroll
-    for (int k = 0; k < VEC; ++k) {
-        const uint32_t i = (uint32_t)k * WAVES + wave;  // uniform
-        uint32_t v = __builtin_amdgcn_readlane(ex, (int)(i / R));
-        if (R == 2 && (i & 1u)) v += __builtin_amdgcn_readlane(e[0], (int)(i / R));
-        prefix[k] = v;
-    }
-    tile_total = readlane(s, 63);
 }
 
 }  // namespace gps
