@@ -11,7 +11,9 @@ progress goes to stderr. The CPU baseline is the serial host scan (the
 reference's validator loop restated in C) timed on this machine's host.
 
 Extra legs (reported under "extras"): csdl_exclusive, rts, csdldf (stall
-periods 0/7/8, one workgroup per tile and persistent), copy (the read-n + write-n ceiling), vendor (rocPRIM's device
+periods 0/7/8, one workgroup per tile and persistent), copy (the read-n + write-n ceiling), ceiling
+(copies and one-way streams of the same 2 GiB with other access patterns, the
+runtime's copy and fill), vendor (rocPRIM's device
 scan, the reference's CUB comparator on ROCm), n2p20 (latency at 2^20),
 segmented (4096 log-normal segments: one graph-launched pass vs one launch
 per segment), segmented_fp (the same segments over FP32: one segmented FP
@@ -34,7 +36,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
-ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "ceiling", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
             "struct", "occupancy", "survey", "tuner"]
 
 
@@ -145,6 +147,45 @@ class Legs:
         e1.record(st)
         torch.cuda.synchronize()
         out["copy_ceiling_gbps"] = round(gbps(self.n, e0.elapsed_time(e1) / 20), 1)
+
+    def ceiling(self, out):
+        """The memory-ceiling probes on the scan's 2 GiB of traffic: copies of
+        the 1 GiB input, and read-only / write-only streams of 2 GiB."""
+        from gpuprefixsums_amd import ceiling
+        torch = self.torch
+        st = torch.cuda.current_stream()
+        nbytes = self.n * 4
+        big = torch.empty(2 * nbytes // 4, dtype=torch.int32, device="cuda")
+        need = max(ceiling.scratch_bytes(k, nbytes) for k in range(ceiling.KINDS))
+        scratch = torch.zeros(need // 4, dtype=torch.int32, device="cuda")
+        res = {}
+        for kind in range(ceiling.KINDS):
+            if kind in ceiling.COPY_KINDS:
+                src, dst, moved = self.d_in, self.d_out, nbytes
+            elif kind == ceiling.READ_1024X1:
+                src, dst, moved = big, self.d_out, 2 * nbytes
+            else:
+                src, dst, moved = None, big, 2 * nbytes
+            for _ in range(3):
+                ceiling.run(kind, src, dst, moved, scratch)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(20):
+                ceiling.run(kind, src, dst, moved, scratch)
+            e1.record(st)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 20
+            traffic = 2 * nbytes  # a copy reads and writes nbytes; a stream moves 2 * nbytes one way
+            entry = {"gbps": round(traffic / (ms * 1e-3) / 1e9, 1),
+                     "frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+            if kind in ceiling.COPY_KINDS:
+                entry["exact"] = bool(torch.equal(self.d_out, self.d_in))
+                self.d_out.zero_()
+            res[ceiling.name(kind)] = entry
+        if int(scratch[0].item()) != 0:
+            raise RuntimeError("ticket word not re-armed")
+        del big, scratch
+        out["ceiling_probes_2GiB"] = res
 
     def vendor(self, out):
         """rocPRIM's device scan (the reference's CUB comparator on ROCm) on the

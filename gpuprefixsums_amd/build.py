@@ -3,8 +3,9 @@
 Compiles the HIP/C++ sources under csrc/ with hipcc for gfx950 into
 gpuprefixsums_amd/libgps.so (the product library), the vendor comparator
 gpuprefixsums_amd/libgps_vendor.so (rocPRIM's scan, for tests and benchmarks
-only; it links libgps.so), the reference-style test program
-gpuprefixsums_amd/gps_main, and the oracle (oracle/liboracle.so, test
+only; it links libgps.so), the memory-ceiling probes
+gpuprefixsums_amd/libgps_ceiling.so (benchmark only; it links libgps.so), the
+reference-style test program gpuprefixsums_amd/gps_main, and the oracle (oracle/liboracle.so, test
 infrastructure) with make.
 
     python -m gpuprefixsums_amd.build          # build everything
@@ -25,6 +26,7 @@ INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libgps.so"
 VENDOR_LIB = PKG / "libgps_vendor.so"
+CEILING_LIB = PKG / "libgps_ceiling.so"
 MAIN = PKG / "gps_main"
 ARCH = "gfx950"
 
@@ -47,6 +49,11 @@ LIB_SOURCES = [
 VENDOR_SOURCES = [
     "vendor_rocprim.hip",
     "vendor_dispatcher.cpp",
+]
+
+# The memory-ceiling probes the benchmark reads the scan's roofline against.
+CEILING_SOURCES = [
+    "ceiling.hip",
 ]
 
 COMMON_FLAGS = [
@@ -99,11 +106,13 @@ def _link(objs, out: Path, extra=()) -> Path:
 
 def build_lib(jobs: int = 8) -> Path:
     BUILD.mkdir(exist_ok=True)
-    sources = [s for s in LIB_SOURCES + VENDOR_SOURCES if (CSRC / s).exists()]
+    sources = [s for s in LIB_SOURCES + VENDOR_SOURCES + CEILING_SOURCES if (CSRC / s).exists()]
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(sources)))) as ex:
         objs = dict(zip(sources, ex.map(_compile, sources)))
     _link([objs[s] for s in LIB_SOURCES], LIB)
-    _link([objs[s] for s in VENDOR_SOURCES], VENDOR_LIB, [f"-L{PKG}", "-lgps", "-Wl,-rpath,$ORIGIN"])
+    uses_lib = [f"-L{PKG}", "-lgps", "-Wl,-rpath,$ORIGIN"]
+    _link([objs[s] for s in VENDOR_SOURCES], VENDOR_LIB, uses_lib)
+    _link([objs[s] for s in CEILING_SOURCES], CEILING_LIB, uses_lib)
     return LIB
 
 

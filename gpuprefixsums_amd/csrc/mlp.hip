@@ -6,12 +6,16 @@
 // array [W1 (H x D, row-major), b1 (H), W2 (H x H), b2 (H), W3 (H), b3 (1)].
 // Layers are a few thousand multiply-adds per sample, so every kernel is a
 // plain one-thread-per-output loop (deterministic summation order, no
-// atomics); matrix cores would not pay at this size.
+// atomics); matrix cores would not pay at this size. Weight gradients sum
+// over the batch, so they are split into fixed chunks of samples whose
+// partial sums are added in chunk order: thousands of threads instead of
+// one per weight, and the same order on every run.
 //
 // No reference counterpart (an extension beside the reference's scans). Compiled with
 // -ffp-contract=off so results are reproducible against the CPU oracle.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -69,21 +73,41 @@ __global__ void sum_partials(const float* partial, uint32_t n, float scale, floa
     }
 }
 
-// dW[o, i] = sum_b dZ[b, o] X[b, i];  db[o] = sum_b dZ[b, o]  (fixed b order).
-__global__ void linear_bwd_w(const float* dZ, const float* X, float* dW, float* db, uint32_t B, uint32_t Din,
-                             uint32_t Dout) {
+// Weight gradients in two deterministic passes. The batch is cut into chunks
+// of kChunk samples; pass 1 writes, per chunk c and output t,
+//   part[c, t] = sum_{b in chunk c} dZ[b, o] X[b, i]   (t = o Din + i), or
+//   part[c, t] = sum_{b in chunk c} dZ[b, o]           (t = Dout Din + o, the bias),
+// in increasing b; pass 2 sums the chunks of each output in increasing c.
+// The summation order is fixed, so results do not depend on scheduling.
+constexpr uint32_t kChunk = 128;
+
+inline uint32_t chunks_of(uint32_t B) { return (B + kChunk - 1) / kChunk; }
+
+__global__ void linear_bwd_w_part(const float* dZ, const float* X, float* part, uint32_t B, uint32_t Din,
+                                  uint32_t Dout) {
+    const uint64_t outs = (uint64_t)Dout * Din + Dout;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= outs) return;
+    const uint32_t c = blockIdx.y;
+    const uint32_t b0 = c * kChunk, b1 = min(B, b0 + kChunk);
+    float acc = 0.f;
     if (t < (uint64_t)Dout * Din) {
         const uint32_t o = (uint32_t)(t / Din), i = (uint32_t)(t % Din);
-        float acc = 0.f;
-        for (uint32_t b = 0; b < B; ++b) acc += dZ[(uint64_t)b * Dout + o] * X[(uint64_t)b * Din + i];
-        dW[t] = acc;
-    } else if (t < (uint64_t)Dout * Din + Dout) {
+        for (uint32_t b = b0; b < b1; ++b) acc += dZ[(uint64_t)b * Dout + o] * X[(uint64_t)b * Din + i];
+    } else {
         const uint32_t o = (uint32_t)(t - (uint64_t)Dout * Din);
-        float acc = 0.f;
-        for (uint32_t b = 0; b < B; ++b) acc += dZ[(uint64_t)b * Dout + o];
-        db[o] = acc;
+        for (uint32_t b = b0; b < b1; ++b) acc += dZ[(uint64_t)b * Dout + o];
     }
+    part[(uint64_t)c * outs + t] = acc;
+}
+
+// dW (Dout x Din) and db (Dout) follow each other in the flat gradient array.
+__global__ void linear_bwd_w_sum(const float* part, float* dWb, uint64_t outs, uint32_t chunks) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= outs) return;
+    float acc = 0.f;
+    for (uint32_t c = 0; c < chunks; ++c) acc += part[(uint64_t)c * outs + t];
+    dWb[t] = acc;
 }
 
 // dZprev[b, i] = (sum_o dZ[b, o] W[o, i]) * (Zprev[b, i] > 0)   (through relu).
@@ -120,6 +144,7 @@ struct gps_mlp {
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
     float *x = nullptr, *t = nullptr, *z1 = nullptr, *a1 = nullptr, *z2 = nullptr, *a2 = nullptr, *y = nullptr;
     float *dy = nullptr, *dz2 = nullptr, *dz1 = nullptr, *partial = nullptr, *loss = nullptr;
+    float* wpart = nullptr;  // per-chunk weight-gradient partial sums
 
     float* W1() const { return params; }
     float* b1() const { return params + (uint64_t)H * D; }
@@ -141,8 +166,8 @@ uint64_t param_count(uint32_t D, uint32_t H) { return (uint64_t)H * D + H + (uin
 
 void free_mlp(gps_mlp* m) {
     if (!m) return;
-    float* ptrs[] = {m->params, m->grads, m->m, m->v, m->x, m->t, m->z1, m->a1, m->z2, m->a2,
-                     m->y, m->dy, m->dz2, m->dz1, m->partial, m->loss};
+    float* ptrs[] = {m->params, m->grads, m->m, m->v,  m->x,       m->t,    m->z1,   m->a1, m->z2,
+                     m->a2,     m->y,     m->dy, m->dz2, m->dz1, m->partial, m->loss, m->wpart};
     for (float* p : ptrs) (void)hipFree(p);
     delete m;
 }
@@ -158,27 +183,34 @@ hipError_t forward(gps_mlp* m, uint32_t B) {
     return hipGetLastError();
 }
 
-// Loss (device scalar) and all parameter gradients for the batch in m->x / m->t.
-hipError_t loss_and_grads(gps_mlp* m, uint32_t B) {
+// dW and db of one layer (Din inputs, Dout outputs) into the flat gradient
+// array at dWb, through the per-chunk partials.
+void linear_bwd_w(gps_mlp* m, const float* dZ, const float* X, float* dWb, uint32_t B, uint32_t Din, uint32_t Dout) {
+    const uint64_t outs = (uint64_t)Dout * Din + Dout;
+    const uint32_t chunks = chunks_of(B);
+    const dim3 g1((uint32_t)((outs + kThreads - 1) / kThreads), chunks);
+    hipLaunchKernelGGL(linear_bwd_w_part, g1, dim3(kThreads), 0, nullptr, dZ, X, m->wpart, B, Din, Dout);
+    hipLaunchKernelGGL(linear_bwd_w_sum, grid_of(outs), dim3(kThreads), 0, nullptr, m->wpart, dWb, outs, chunks);
+}
+
+// Loss (device scalar at `loss`) and all parameter gradients for the batch in m->x / m->t.
+hipError_t loss_and_grads(gps_mlp* m, uint32_t B, float* loss) {
     const uint32_t D = m->D, H = m->H;
     hipError_t e = forward(m, B);
     if (e != hipSuccess) return e;
     const dim3 gb = grid_of(B);
     hipLaunchKernelGGL(mse_grad, gb, dim3(kThreads), 0, nullptr, m->y, m->t, m->dy, m->partial, B);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(64), 0, nullptr, m->partial, gb.x, 1.f / (float)B, m->loss);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(64), 0, nullptr, m->partial, gb.x, 1.f / (float)B, loss);
     // layer 3: dz3 = dy
-    hipLaunchKernelGGL(linear_bwd_w, grid_of((uint64_t)H + 1), dim3(kThreads), 0, nullptr, m->dy, m->a2, m->gW3(),
-                       m->gb3(), B, H, 1u);
+    linear_bwd_w(m, m->dy, m->a2, m->gW3(), B, H, 1u);
     hipLaunchKernelGGL(linear_bwd_x, grid_of((uint64_t)B * H), dim3(kThreads), 0, nullptr, m->dy, m->W3(), m->z2,
                        m->dz2, B, H, 1u);
     // layer 2
-    hipLaunchKernelGGL(linear_bwd_w, grid_of((uint64_t)H * H + H), dim3(kThreads), 0, nullptr, m->dz2, m->a1,
-                       m->gW2(), m->gb2(), B, H, H);
+    linear_bwd_w(m, m->dz2, m->a1, m->gW2(), B, H, H);
     hipLaunchKernelGGL(linear_bwd_x, grid_of((uint64_t)B * H), dim3(kThreads), 0, nullptr, m->dz2, m->W2(), m->z1,
                        m->dz1, B, H, H);
     // layer 1 (no gradient into x)
-    hipLaunchKernelGGL(linear_bwd_w, grid_of((uint64_t)H * D + H), dim3(kThreads), 0, nullptr, m->dz1, m->x,
-                       m->gW1(), m->gb1(), B, D, H);
+    linear_bwd_w(m, m->dz1, m->x, m->gW1(), B, D, H);
     return hipGetLastError();
 }
 
@@ -215,7 +247,8 @@ int gps_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t max_batch, gps_mlp
     } allocs[] = {{&m->params, m->count}, {&m->grads, m->count}, {&m->m, m->count},    {&m->v, m->count},
                   {&m->x, B * in_dim},    {&m->t, B},            {&m->z1, B * hidden}, {&m->a1, B * hidden},
                   {&m->z2, B * hidden},   {&m->a2, B * hidden},  {&m->y, B},           {&m->dy, B},
-                  {&m->dz2, B * hidden},  {&m->dz1, B * hidden}, {&m->partial, nblocks}, {&m->loss, 1}};
+                  {&m->dz2, B * hidden},  {&m->dz1, B * hidden}, {&m->partial, nblocks}, {&m->loss, 1},
+                  {&m->wpart, ((uint64_t)hidden * std::max(in_dim, hidden) + hidden) * chunks_of(max_batch)}};
     for (auto& a : allocs) {
         const hipError_t e = hipMalloc(a.p, a.n * sizeof(float));
         if (e != hipSuccess) {
@@ -269,7 +302,7 @@ int gps_mlp_loss_grad(gps_mlp* m, const float* h_x, const float* h_t, uint32_t b
     if (!h_t) return fail(GPS_ERR_INVALID, "null targets");
     int rc = stage_batch(m, h_x, h_t, batch);
     if (rc) return rc;
-    HIP_TRY(loss_and_grads(m, batch));
+    HIP_TRY(loss_and_grads(m, batch, m->loss));
     if (h_loss) HIP_TRY(hipMemcpy(h_loss, m->loss, sizeof(float), hipMemcpyDeviceToHost));
     if (h_grad) HIP_TRY(hipMemcpy(h_grad, m->grads, m->count * sizeof(float), hipMemcpyDeviceToHost));
     HIP_TRY(hipDeviceSynchronize());
@@ -282,18 +315,26 @@ int gps_mlp_train(gps_mlp* m, const float* h_x, const float* h_t, uint32_t batch
     if (rc) return rc;
     if (!h_t) return fail(GPS_ERR_INVALID, "null targets");
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-    std::vector<float> curve;
-    for (uint32_t ep = 0; ep < epochs; ++ep) {
-        HIP_TRY(loss_and_grads(m, batch));
+    // The epoch losses stay on the device until the end (one copy, no
+    // per-epoch synchronisation).
+    float* d_curve = nullptr;
+    if (h_loss_curve && epochs) HIP_TRY(hipMalloc(&d_curve, (size_t)epochs * sizeof(float)));
+    hipError_t e = hipSuccess;
+    for (uint32_t ep = 0; ep < epochs && e == hipSuccess; ++ep) {
+        e = loss_and_grads(m, batch, d_curve ? d_curve + ep : m->loss);
+        if (e != hipSuccess) break;
         ++m->step;
         const float bc1 = 1.f - std::pow(beta1, (float)m->step);
         const float bc2 = 1.f - std::pow(beta2, (float)m->step);
         hipLaunchKernelGGL(adam_step, grid_of(m->count), dim3(kThreads), 0, nullptr, m->params, m->grads, m->m,
                            m->v, m->count, lr, beta1, beta2, eps, weight_decay, bc1, bc2);
-        HIP_TRY(hipGetLastError());
-        if (h_loss_curve) HIP_TRY(hipMemcpy(&h_loss_curve[ep], m->loss, sizeof(float), hipMemcpyDeviceToHost));
+        e = hipGetLastError();
     }
-    HIP_TRY(hipDeviceSynchronize());
+    if (e == hipSuccess && d_curve)
+        e = hipMemcpy(h_loss_curve, d_curve, (size_t)epochs * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (d_curve) (void)hipFree(d_curve);
+    if (e != hipSuccess) return hip_fail(e, "gps_mlp_train");
     return GPS_OK;
 }
 

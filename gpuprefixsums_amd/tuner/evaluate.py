@@ -2,9 +2,11 @@
 
     python -m gpuprefixsums_amd.tuner.evaluate --times times.jsonl [--save]
 
-Training set: the first 384 workloads of the C5 sweep and the first 72 of
-the decoupled-fallback (stall) sweep; held out: the last 128 and the last 24.
-One model is trained on both and judged on each held-out set separately.
+Training set: the first 384 workloads of the C5 sweep, the 1024 workloads of
+the extra training sweep (same distribution, another seed; optional) and the
+first 72 of the decoupled-fallback (stall) sweep; held out: the last 128 of
+the C5 sweep and the last 24 of the stall sweep. One model is trained on all
+of it and judged on each held-out set separately.
 For a held-out workload with exhaustively measured times T(c):
     regret(choice) = T(choice) / min_c T(c) - 1
 reported as mean / median / max over the held-out set, for the tuner's
@@ -12,11 +14,12 @@ choice and for two methods without the model: the library default (a default
 handle's per-size choice), and the best static rule (per dtype, the one
 configuration with the lowest mean regret on the training workloads).
 
-The tuner takes the model's fastest predicted configuration only when it
-beats the default's predicted time by more than a margin. The margin and the
-training hyper-parameters (weight decay, epochs) are chosen by 4-fold cross-
-validation on the training workloads alone; the held-out workloads judge
-the result once.
+The model is an ensemble of ENSEMBLE MLPs trained from different seeds, its
+prediction their mean. The tuner takes the model's fastest predicted
+configuration only when it beats the default's predicted time by more than a
+margin. The margin and the training hyper-parameters (weight decay, epochs)
+are chosen by 4-fold cross-validation on the training workloads alone; the
+held-out workloads judge the result once.
 """
 from __future__ import annotations
 
@@ -25,16 +28,17 @@ import json
 import math
 import sys
 from pathlib import Path
-from typing import Dict, List, Tuple
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
 from . import DATA, DEFAULT_MODEL
 from .model import CostModel
-from .space import (Config, Workload, c5_workloads, candidates, default_config, features, fixed_default_config,
-                    gated_choice, split, stall_workloads)
+from .space import (Config, Workload, c5_workloads, candidates, default_config, extra_workloads, features,
+                    fixed_default_config, gated_choice, split, stall_workloads)
 
 CV_FOLDS = 4
+ENSEMBLE = 5  # MLPs per model (seeds 0..4), predictions averaged
 CV_WEIGHT_DECAY = (1e-5, 1e-4, 1e-3)
 CV_EPOCHS = (2000, 4000)
 CV_MARGINS = (0.0, 0.02, 0.05, 0.1, 0.2)
@@ -117,7 +121,9 @@ def cross_validate(train: List[Workload], times: Dict[int, Dict[str, float]], se
             for f in range(CV_FOLDS):
                 fit_ws = [w for g in range(CV_FOLDS) if g != f for w in folds[g]]
                 X, y = dataset(fit_ws, times)
-                model = CostModel().fit(X, y, epochs=ep, weight_decay=wd, seed=seed)
+                model = CostModel(members=ENSEMBLE).fit(X, y, epochs=ep, weight_decay=wd, seed=seed)
+                print(f"[cv] weight decay {wd:g}, epochs {ep}, fold {f + 1}/{CV_FOLDS} fitted on {len(X)} samples",
+                      file=sys.stderr, flush=True)
                 for w in folds[f]:
                     t = times[w.wid]
                     best = min(t.values())
@@ -133,13 +139,23 @@ def cross_validate(train: List[Workload], times: Dict[int, Dict[str, float]], se
             "grid": {f"wd={k[0]:g} epochs={k[1]} margin={k[2]:g}": v for k, v in scores.items()}}
 
 
-def load_sweeps(c5_path: Path, stall_path: Path):
-    """Timings of both sweeps (the stall file is optional) and their
-    train / held-out splits: (times, c5 train, c5 held, stall train, stall held)."""
+EXTRA_TIMES = "c5x_times_mi355x.jsonl"
+
+
+def load_sweeps(c5_path: Path, stall_path: Path, extra_path: Optional[Path] = None):
+    """Timings of the sweeps and their train / held-out splits:
+    (times, c5 train, c5 held, stall train, stall held). The stall file is
+    optional; so is the extra training sweep (by default the file of that
+    name beside c5_path), whose workloads all join the c5 training set."""
     times = load_times(c5_path)
     if stall_path and Path(stall_path).exists():
         times.update(load_times(stall_path))
+    if extra_path is None:
+        extra_path = Path(c5_path).with_name(EXTRA_TIMES)
+    if Path(extra_path).exists():
+        times.update(load_times(extra_path))
     train, held = split([w for w in c5_workloads() if w.wid in times])
+    train = train + [w for w in extra_workloads() if w.wid in times]
     s_train, s_held = split([w for w in stall_workloads() if w.wid in times])
     return times, train, held, s_train, s_held
 
@@ -162,7 +178,8 @@ def main(argv=None) -> int:
     print(f"cross-validation: weight decay {cv['weight_decay']:g}, epochs {cv['epochs']}, "
           f"margin {cv['margin']:g}, regret {cv['cv_regret']:.4f}", file=sys.stderr)
     X, y = dataset(train, times)
-    model = CostModel().fit(X, y, epochs=cv["epochs"], lr=args.lr, weight_decay=cv["weight_decay"], seed=args.seed)
+    model = CostModel(members=ENSEMBLE).fit(X, y, epochs=cv["epochs"], lr=args.lr, weight_decay=cv["weight_decay"],
+                                            seed=args.seed)
     model.margin = cv["margin"]
     Xh, yh = dataset(held, times)
     rmse_train = float(np.sqrt(np.mean((model.predict(X) - y) ** 2)))

@@ -3,7 +3,7 @@ workload on the GPU. Its output is both the cost model's training data and
 the judge of the model's regret; for a workload the model does not cover it
 is the fallback.
 
-    python -m gpuprefixsums_amd.tuner.harness --out times.jsonl [--sweep c5|stall] [--start 0 --count 512]
+    python -m gpuprefixsums_amd.tuner.harness --out times.jsonl [--sweep c5|stall|extra] [--start 0 --count 512]
                                               [--configs name,name,...]
 
 One JSON line per workload: {"wid", "n", "dtype", "segments", "times_ms": {config: ms}}.
@@ -20,7 +20,7 @@ import time
 from pathlib import Path
 from typing import Dict
 
-from .space import Config, Workload, c5_workloads, candidates, stall_workloads
+from .space import Config, Workload, c5_workloads, candidates, extra_workloads, stall_workloads
 
 
 class Runner:
@@ -114,14 +114,16 @@ class Runner:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sweep", choices=("c5", "stall"), default="c5")
+    ap.add_argument("--sweep", choices=("c5", "stall", "extra"), default="c5")
     ap.add_argument("--start", type=int, default=0)
-    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--count", type=int, default=1 << 20, help="time at most this many workloads")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--configs", default=None, help="comma list: time only these configurations")
     args = ap.parse_args(argv)
     if args.sweep == "c5":
         ws = c5_workloads(6 if args.seed is None else args.seed)
+    elif args.sweep == "extra":
+        ws = extra_workloads(8 if args.seed is None else args.seed)
     else:
         ws = stall_workloads(7 if args.seed is None else args.seed)
     ws = ws[args.start:args.start + args.count]

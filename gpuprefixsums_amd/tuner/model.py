@@ -1,5 +1,6 @@
-"""The learned cost model: a 3-layer MLP (hidden 64, FP32) over workload and
-configuration features that predicts log2 of the run time in microseconds.
+"""The learned cost model: an ensemble of 3-layer MLPs (hidden 64, FP32) over
+workload and configuration features that predicts log2 of the run time in
+microseconds.
 Training and inference run on the GPU through the library's own HIP kernels
 (gps_mlp_* in include/gps.h); this module only standardises features and
 moves small host arrays.
@@ -73,15 +74,22 @@ class GpuMlp:
 
 
 class CostModel:
-    """Standardised features -> GpuMlp -> predicted log2(us)."""
+    """Standardised features -> an ensemble of GpuMlp -> predicted log2(us).
 
-    def __init__(self, hidden: int = 64):
+    `members` MLPs are trained from different seeds on the same data and
+    their predictions averaged: one network's fit varies with its
+    initialisation by more than many configurations differ in time, and the
+    mean of several is steadier where the tuner must choose between near ties.
+    """
+
+    def __init__(self, hidden: int = 64, members: int = 1):
         self.hidden = hidden
+        self.members = members
         self.margin = 0.0  # the tuner's gate: deviate from the default only above this predicted gain
         self.mean: Optional[np.ndarray] = None
         self.std: Optional[np.ndarray] = None
-        self.params: Optional[np.ndarray] = None
-        self.curve: Optional[np.ndarray] = None
+        self.params: Optional[np.ndarray] = None  # (members, count)
+        self.curve: Optional[np.ndarray] = None  # the first member's training loss per epoch
 
     def _norm(self, X: np.ndarray) -> np.ndarray:
         return (np.asarray(X, np.float64) - self.mean) / self.std
@@ -97,25 +105,34 @@ class CostModel:
 
     def fit(self, X: np.ndarray, y: np.ndarray, epochs: int = 4000, lr: float = 3e-3, weight_decay: float = 1e-5,
             seed: int = 0) -> "CostModel":
-        """Full-batch AdamW on the GPU (the library's forward/backward/Adam kernels)."""
+        """Full-batch AdamW on the GPU (the library's forward/backward/Adam
+        kernels), once per member with seeds seed, seed + 1, ..."""
         X = np.asarray(X, np.float64)
         self.mean = X.mean(axis=0)
         self.std = np.where(X.std(axis=0) > 1e-12, X.std(axis=0), 1.0)
         m = GpuMlp(X.shape[1], self.hidden, len(X))
+        params = []
         try:
-            m.set_params(self.init_params(X.shape[1], self.hidden, seed))
-            self.curve = m.train(self._norm(X), np.asarray(y, np.float32), epochs, lr, weight_decay)
-            self.params = m.get_params()
+            for k in range(self.members):
+                m.set_params(self.init_params(X.shape[1], self.hidden, seed + k))
+                curve = m.train(self._norm(X), np.asarray(y, np.float32), epochs, lr, weight_decay)
+                if k == 0:
+                    self.curve = curve
+                params.append(m.get_params())
         finally:
             m.close()
+        self.params = np.stack(params)
         return self
 
     def predict(self, X: np.ndarray) -> np.ndarray:
         assert self.params is not None, "model not trained or loaded"
         m = GpuMlp(self.mean.size, self.hidden, max(1, len(X)))
         try:
-            m.set_params(self.params)
-            return m.forward(self._norm(X)).astype(np.float64)
+            out = np.zeros(len(X), np.float64)
+            for p in self.params:
+                m.set_params(p)
+                out += m.forward(self._norm(X)).astype(np.float64)
+            return out / len(self.params)
         finally:
             m.close()
 
@@ -126,8 +143,9 @@ class CostModel:
     @classmethod
     def load(cls, path: Path) -> "CostModel":
         d = np.load(path, allow_pickle=False)
-        m = cls(int(d["hidden"]))
-        m.params, m.mean, m.std = d["params"], d["mean"], d["std"]
+        params = np.atleast_2d(d["params"])
+        m = cls(int(d["hidden"]), members=params.shape[0])
+        m.params, m.mean, m.std = params, d["mean"], d["std"]
         m.curve = d["curve"]
         m.margin = float(d["margin"]) if "margin" in d.files else 0.0
         return m

@@ -182,7 +182,7 @@ def lognormal_partition(n: int, segments: int, rng: np.random.Generator, sigma: 
     return np.concatenate([[0], np.cumsum(lengths)]).astype(np.uint64)
 
 
-def c5_workloads(seed: int = 6, count: int = 512) -> List[Workload]:
+def c5_workloads(seed: int = 6, count: int = 512, wid_base: int = 0) -> List[Workload]:
     """The tuner sweep: n log-uniform in [2^10, 2^30], dtype uniform over
     {u32, f32, bf16}, segment count log-uniform in [1, 2^14]."""
     rng = np.random.default_rng(seed)
@@ -191,8 +191,18 @@ def c5_workloads(seed: int = 6, count: int = 512) -> List[Workload]:
         n = int(round(2.0 ** rng.uniform(10, 30)))
         dtype = DTYPES[int(rng.integers(3))]
         segments = int(min(max(round(2.0 ** rng.uniform(0, 14)), 1), 1 << 14))
-        out.append(Workload(i, n, dtype, segments, lognormal_partition(n, segments, rng)))
+        out.append(Workload(wid_base + i, n, dtype, segments, lognormal_partition(n, segments, rng)))
     return out
+
+
+EXTRA_WID_BASE = 20000
+
+
+def extra_workloads(seed: int = 8, count: int = 1024) -> List[Workload]:
+    """More training workloads from the same distribution as the tuner sweep
+    (another seed; workload ids start at 20000). All of them train; the tuner
+    sweep's held-out workloads stay the only judge."""
+    return c5_workloads(seed, count, EXTRA_WID_BASE)
 
 
 STALL_WID_BASE = 10000

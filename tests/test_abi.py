@@ -12,8 +12,10 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 
 
-# The comparator library's header; every other include/*.h belongs to libgps.so.
+# The comparator library's header and the ceiling probes' header; every other
+# include/*.h belongs to libgps.so.
 VENDOR_HEADER = "gps_vendor.h"
+CEILING_HEADER = "gps_ceiling.h"
 
 
 def declared_in(h: Path) -> set:
@@ -25,13 +27,17 @@ def declared_functions() -> set:
     """Entry points of the product library libgps.so."""
     names = set()
     for h in (ROOT / "include").glob("*.h"):
-        if h.name != VENDOR_HEADER:
+        if h.name not in (VENDOR_HEADER, CEILING_HEADER):
             names |= declared_in(h)
     return names
 
 
 def vendor_functions() -> set:
     return declared_in(ROOT / "include" / VENDOR_HEADER)
+
+
+def ceiling_functions() -> set:
+    return declared_in(ROOT / "include" / CEILING_HEADER)
 
 
 def test_header_declares_entry_points():
@@ -46,10 +52,11 @@ def test_library_exports_every_declared_symbol(gps_lib):
 
 
 def test_python_binding_covers_header():
-    from gpuprefixsums_amd import _lib, vendor
+    from gpuprefixsums_amd import _lib, ceiling, vendor
     assert declared_functions() == set(_lib.SIGNATURES), (
         declared_functions() ^ set(_lib.SIGNATURES))
     assert vendor_functions() == set(vendor.SIGNATURES), vendor_functions() ^ set(vendor.SIGNATURES)
+    assert ceiling_functions() == set(ceiling.SIGNATURES), ceiling_functions() ^ set(ceiling.SIGNATURES)
 
 
 def test_comparator_library_exports_its_header(gps_lib):
@@ -59,6 +66,49 @@ def test_comparator_library_exports_its_header(gps_lib):
     v = vendor.load()
     missing = [n for n in sorted(vendor_functions()) if not hasattr(v, n)]
     assert not missing, f"libgps_vendor.so lacks {missing}"
+
+
+def test_ceiling_library_exports_its_header(gps_lib):
+    from gpuprefixsums_amd import build, ceiling
+    if not ceiling.CEILING_LIB_PATH.exists():
+        build.build_lib()
+    c = ceiling.load()
+    missing = [n for n in sorted(ceiling_functions()) if not hasattr(c, n)]
+    assert not missing, f"libgps_ceiling.so lacks {missing}"
+    assert not any(hasattr(gps_lib, n) for n in ceiling_functions())
+
+
+def test_ceiling_probe_metadata_and_argument_errors(gps_lib):
+    """Names, granules and argument checks that fail before any HIP call."""
+    from gpuprefixsums_amd import ceiling
+    from gpuprefixsums_amd._lib import GPS_ERR_INVALID, GpsError
+    header = (ROOT / "include" / CEILING_HEADER).read_text()
+    assert f"GPS_CEIL_KINDS = {ceiling.KINDS}" in header
+    names = [ceiling.name(k) for k in range(ceiling.KINDS)]
+    assert len(set(names)) == ceiling.KINDS and ceiling.name(ceiling.KINDS) == "unknown"
+    assert ceiling.granule(ceiling.COPY_64X1) == 1024
+    assert ceiling.granule(ceiling.COPY_1024X8) == 128 * 1024
+    assert ceiling.granule(ceiling.TICKET_1024X4) == 64 * 1024
+    with pytest.raises(GpsError) as e:
+        ceiling.granule(ceiling.KINDS)
+    assert e.value.code == GPS_ERR_INVALID
+    fake = 1 << 20  # never dereferenced: every call below fails its argument check first
+    for kind, nbytes, src, dst in ((ceiling.COPY_1024X8, 1024, fake, fake),        # not a whole tile
+                                   (ceiling.COPY_64X1, 0, fake, fake),             # empty
+                                   (ceiling.COPY_64X1, 1024, 0, fake),             # null source
+                                   (ceiling.WRITE_256X1, 4096, 0, 0),              # null destination
+                                   (ceiling.COPY_64X1, 1024, fake + 4, fake)):     # misaligned
+        with pytest.raises(GpsError) as e:
+            ceiling.run(kind, src, dst, nbytes, stream=0)
+        assert e.value.code == GPS_ERR_INVALID
+    for kind in ceiling.TICKETED:  # no scratch, or scratch off its 256-byte alignment
+        for scratch in (None, fake + 128):
+            with pytest.raises(GpsError) as e:
+                ceiling.run(kind, fake, fake, ceiling.granule(kind), scratch=scratch, stream=0)
+            assert e.value.code == GPS_ERR_INVALID
+    assert ceiling.scratch_bytes(ceiling.COPY_64X1, 1 << 20) == 0
+    assert ceiling.scratch_bytes(ceiling.TICKET_1024X4, 1 << 20) == 256
+    assert ceiling.scratch_bytes(ceiling.TICKET_1024X4_STATUS, 1 << 20) == 256 + 16 * 128
 
 
 def test_product_library_carries_no_vendor_scan(gps_lib):
