@@ -15,9 +15,10 @@ periods 0/7/8, one workgroup per tile and persistent), copy (the read-n + write-
 (copies and one-way streams of the same 2 GiB with other access patterns, the
 runtime's copy and fill), vendor (rocPRIM's device
 scan, the reference's CUB comparator on ROCm), n2p20 (latency at 2^20),
-segmented (4096 log-normal segments: one graph-launched pass vs one launch
-per segment), segmented_fp (the same segments over FP32: one segmented FP
-launch vs one launch per segment), fp (n = 2^27 FP32/BF16 with and without compensated carries,
+segmented (4096 log-normal segments in one graph-launched pass) and
+segmented_loop (the same as one launch per segment), segmented_fp and
+segmented_fp_loop (the same segments over FP32: one segmented FP launch, and
+one launch per segment), fp (n = 2^27 FP32/BF16 with and without compensated carries,
 errors vs FP64), tuner (held-out regret of the learned cost model).
 """
 from __future__ import annotations
@@ -36,7 +37,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 BYTES_PER_ELEM = 8
-ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "ceiling", "vendor", "n2p20", "segmented", "segmented_fp", "fp",
+ALL_LEGS = ["csdl_exclusive", "rts", "csdldf", "copy", "ceiling", "vendor", "n2p20", "segmented", "segmented_loop",
+            "segmented_fp", "segmented_fp_loop", "fp",
             "struct", "occupancy", "survey", "tuner"]
 
 
@@ -203,100 +205,120 @@ class Legs:
         out["rocprim_inclusive"] = {"gbps": round(gbps(self.n, mean), 1), "kernel_ms_mean": round(mean, 5),
                                     "validation_errors": errors, "temp_bytes": tb}
 
-    def segmented(self, out):
-        """4096 segments, log-normal lengths (median 4096, sigma 1.5, clipped to
-        [1, 2^22]), values uniform in [0, 1023], seed 3, exclusive: one launch
-        replayed from a hipGraph vs one plain scan launch per segment."""
-        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+    def _c3_segments(self):
+        """C3's 4096 segments: log-normal lengths (median 4096, sigma 1.5,
+        clipped to [1, 2^22]), seed 3."""
+        oracle, np, torch = self.oracle, self.np, self.torch
         S = 4096
         off = oracle.gen_lognormal_offsets(3, S, median=4096.0, sigma=1.5, lo=1, hi=1 << 22)
-        n = int(off[-1])
-        x = torch.from_numpy(oracle.gen_u32(3, n, 10).view(np.int32)).to("cuda")
-        d_off = torch.from_numpy(off.view(np.int64)).to("cuda")
-        a = torch.empty_like(x)
-        b = torch.empty_like(x)
+        return S, off, int(off[-1]), torch.from_numpy(off.view(np.int64)).to("cuda")
+
+    def _events(self, fn, iters):
+        torch = self.torch
         st = torch.cuda.current_stream()
-        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(iters):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / iters
+
+    def segmented(self, out):
+        """C3: 4096 log-normal segments of values uniform in [0, 1023] (seed 3),
+        exclusive, in one launch: launched directly, and replayed from a
+        hipGraph (the form a serving loop would use)."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        S, off, n, d_off = self._c3_segments()
+        x = torch.from_numpy(oracle.gen_u32(3, n, 10).view(np.int32)).to("cuda")
+        a, b = torch.empty_like(x), torch.empty_like(x)
         with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+            for _ in range(3):
+                h.segmented(x, b, d_off, S, n, False)
+            direct_ms = self._events(lambda: h.segmented(x, b, d_off, S, n, False), 20)
             with h.segmented_graph(x, a, d_off, S, n, False) as g:
                 for _ in range(3):
                     g.launch()
-                e0, e1 = ev(), ev()
-                e0.record(st)
-                for _ in range(20):
-                    g.launch()
-                e1.record(st)
-                torch.cuda.synchronize()
-                graph_ms = e0.elapsed_time(e1) / 20
-            h.segmented_per_segment(x, b, off, False)
-            torch.cuda.synchronize()
-            e0, e1 = ev(), ev()
-            e0.record(st)
-            for _ in range(3):
-                h.segmented_per_segment(x, b, off, False)
-            e1.record(st)
-            torch.cuda.synchronize()
-            loop_ms = e0.elapsed_time(e1) / 3
+                graph_ms = self._events(g.launch, 20)
             if not torch.equal(a, b) or h.device_error():
                 raise RuntimeError("segmented outputs differ")
         nbytes = 8 * n + 8 * (S + 1)
-        out["segmented"] = {
-            "segments": S, "n_total": n, "graph_ms": round(graph_ms, 4), "per_segment_launches_ms": round(loop_ms, 3),
-            "speedup": round(loop_ms / graph_ms, 1), "graph_gbps": round(nbytes / graph_ms / 1e6, 1),
-            "per_segment_gbps": round(nbytes / loop_ms / 1e6, 1), "bytes": nbytes}
+        out["segmented"] = {"segments": S, "n_total": n, "graph_ms": round(graph_ms, 4),
+                            "graph_gbps": round(nbytes / graph_ms / 1e6, 1), "direct_ms": round(direct_ms, 4),
+                            "direct_gbps": round(nbytes / direct_ms / 1e6, 1), "bytes": nbytes}
+
+    def segmented_loop(self, out):
+        """C3's segments as one plain scan launch per segment (the baseline the
+        batched kernel replaces), checked equal to the batched result."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        S, off, n, d_off = self._c3_segments()
+        x = torch.from_numpy(oracle.gen_u32(3, n, 10).view(np.int32)).to("cuda")
+        a, b = torch.empty_like(x), torch.empty_like(x)
+        with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+            h.segmented(x, a, d_off, S, n, False)
+            h.segmented_per_segment(x, b, off, False)
+            torch.cuda.synchronize()
+            loop_ms = self._events(lambda: h.segmented_per_segment(x, b, off, False), 3)
+            if not torch.equal(a, b) or h.device_error():
+                raise RuntimeError("segmented outputs differ")
+        nbytes = 8 * n + 8 * (S + 1)
+        r = {"per_segment_launches_ms": round(loop_ms, 3), "per_segment_gbps": round(nbytes / loop_ms / 1e6, 1)}
+        if "segmented" in out:
+            r["speedup_of_graph"] = round(loop_ms / out["segmented"]["graph_ms"], 1)
+        out["segmented_loop"] = r
 
     def segmented_fp(self, out):
-        """The same 4096 log-normal segments (seed 3) over FP32 ~ N(0,1)
-        values (seed 4), compensated inclusive: one segmented FP launch vs one
-        FP scan launch per segment; errors of both against the FP64 oracle."""
+        """C3's segments over FP32 ~ N(0,1) values (seed 4), compensated
+        inclusive, in one segmented FP launch; errors against the FP64 oracle."""
         gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
-        S = 4096
-        off = oracle.gen_lognormal_offsets(3, S, median=4096.0, sigma=1.5, lo=1, hi=1 << 22)
-        n = int(off[-1])
+        S, off, n, d_off = self._c3_segments()
         xh = oracle.gen_f32_normal(4, n)
         x = torch.from_numpy(xh).to("cuda")
-        d_off = torch.from_numpy(off.view(np.int64)).to("cuda")
         a = torch.empty_like(x)
-        b = torch.empty_like(x)
-        st = torch.cuda.current_stream()
-        ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
         with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
             for _ in range(3):
                 h.segmented_fp(x, a, d_off, S, n)
-            e0, e1 = ev(), ev()
-            e0.record(st)
-            for _ in range(20):
-                h.segmented_fp(x, a, d_off, S, n)
-            e1.record(st)
-            torch.cuda.synchronize()
-            batched_ms = e0.elapsed_time(e1) / 20
-            h.fp_per_segment(x, b, off)
-            torch.cuda.synchronize()
-            e0, e1 = ev(), ev()
-            e0.record(st)
-            for _ in range(3):
-                h.fp_per_segment(x, b, off)
-            e1.record(st)
-            torch.cuda.synchronize()
-            loop_ms = e0.elapsed_time(e1) / 3
+            batched_ms = self._events(lambda: h.segmented_fp(x, a, d_off, S, n), 20)
             if h.device_error():
                 raise RuntimeError("device error in the segmented FP leg")
         y64 = oracle.segmented_scan_fp_f64(xh, off)
         s64 = oracle.segmented_scan_fp_f64(xh, off, absolute=True)
-        err = lambda y: np.abs(y.cpu().numpy().astype(np.float64) - y64)  # noqa: E731
-        ea, eb = err(a), err(b)
-        ra, rb = ea / np.maximum(np.abs(y64), 1.0), eb / np.maximum(np.abs(y64), 1.0)
-        wa, wb = ea / np.maximum(s64, np.finfo(np.float64).tiny), eb / np.maximum(s64, np.finfo(np.float64).tiny)
+        e = np.abs(a.cpu().numpy().astype(np.float64) - y64)
+        rel = e / np.maximum(np.abs(y64), 1.0)
+        norm = e / np.maximum(s64, np.finfo(np.float64).tiny)
         nbytes = 8 * n + 8 * (S + 1)
         out["segmented_fp"] = {
             "segments": S, "n_total": n, "batched_ms": round(batched_ms, 4),
-            "per_segment_launches_ms": round(loop_ms, 3), "speedup": round(loop_ms / batched_ms, 1),
             "batched_gbps": round(nbytes / batched_ms / 1e6, 1),
-            "max_rel_err": {"batched": float(ra.max()), "per_segment": float(rb.max())},
-            "median_rel_err": {"batched": float(np.median(ra)), "per_segment": float(np.median(rb))},
-            "max_normwise_err": {"batched": float(wa.max()), "per_segment": float(wb.max())},
+            "max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
+            "max_normwise_err": float(norm.max()),
             "error_measures": "rel_i = |y_i - y64_i| / max(|y64_i|, 1); normwise_i = |y_i - y64_i| / S_i with "
                               "S_i = sum of |x_j| from the segment head through i"}
+
+    def segmented_fp_loop(self, out):
+        """C3's FP32 segments as one compensated FP scan launch per segment,
+        with the same error measures."""
+        gps, torch, oracle, np = self.gps, self.torch, self.oracle, self.np
+        S, off, n, d_off = self._c3_segments()
+        xh = oracle.gen_f32_normal(4, n)
+        x = torch.from_numpy(xh).to("cuda")
+        b = torch.empty_like(x)
+        with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+            h.fp_per_segment(x, b, off)
+            torch.cuda.synchronize()
+            loop_ms = self._events(lambda: h.fp_per_segment(x, b, off), 3)
+            if h.device_error():
+                raise RuntimeError("device error in the per-segment FP leg")
+        y64 = oracle.segmented_scan_fp_f64(xh, off)
+        s64 = oracle.segmented_scan_fp_f64(xh, off, absolute=True)
+        e = np.abs(b.cpu().numpy().astype(np.float64) - y64)
+        nbytes = 8 * n + 8 * (S + 1)
+        r = {"per_segment_launches_ms": round(loop_ms, 3), "per_segment_gbps": round(nbytes / loop_ms / 1e6, 1),
+             "max_rel_err": float((e / np.maximum(np.abs(y64), 1.0)).max()),
+             "max_normwise_err": float((e / np.maximum(s64, np.finfo(np.float64).tiny)).max())}
+        if "segmented_fp" in out:
+            r["speedup_of_batched"] = round(loop_ms / out["segmented_fp"]["batched_ms"], 1)
+        out["segmented_fp_loop"] = r
 
     def fp(self, out):
         """n = 2^27 inclusive FP scans: FP32 ~ N(0,1) (seed 4) and BF16 with

@@ -1206,7 +1206,7 @@ template <int B, int V, bool BF16, class Ops>
 static hipError_t launch_fp_pp(const FpParams& p, hipStream_t stream) {
     const uint32_t resident = resident_workgroups<scan_fp_pp<B, V, BF16, Ops>, B>();
     if (resident == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    const uint32_t grid = persistent_grid(p.num_tiles, resident);
     hipLaunchKernelGGL((scan_fp_pp<B, V, BF16, Ops>), dim3(grid), dim3(B), 0, stream, p);
     return hipGetLastError();
 }
@@ -1215,7 +1215,7 @@ template <int B, int V, bool BF16, class Ops>
 static hipError_t launch_fp_bl(const FpParams& p, hipStream_t stream) {
     const uint32_t resident = resident_workgroups<scan_fp_bl<B, V, BF16, Ops>, B>();
     if (resident == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    const uint32_t grid = persistent_grid(p.num_tiles, resident);
     hipLaunchKernelGGL((scan_fp_bl<B, V, BF16, Ops>), dim3(grid), dim3(B), 0, stream, p);
     return hipGetLastError();
 }
@@ -1267,12 +1267,12 @@ static hipError_t launch_segfp_pp(const SegFpParams& p, hipStream_t stream) {
     if constexpr (COMP) {
         const uint32_t resident = resident_workgroups<segscan_fp_bl<B, V, BF16, COMP>, B>();
         if (resident == 0) return hipErrorInvalidDevice;
-        const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+        const uint32_t grid = persistent_grid(p.num_tiles, resident);
         hipLaunchKernelGGL((segscan_fp_bl<B, V, BF16, COMP>), dim3(grid), dim3(B), 0, stream, p);
     } else {
         const uint32_t resident = resident_workgroups<segscan_fp_pp<B, V, BF16, COMP>, B>();
         if (resident == 0) return hipErrorInvalidDevice;
-        const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+        const uint32_t grid = persistent_grid(p.num_tiles, resident);
         hipLaunchKernelGGL((segscan_fp_pp<B, V, BF16, COMP>), dim3(grid), dim3(B), 0, stream, p);
     }
     return hipGetLastError();

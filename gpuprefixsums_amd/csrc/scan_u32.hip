@@ -1065,7 +1065,7 @@ template <int B, int V, bool I>
 static hipError_t launch_pp(const ScanParams& p, hipStream_t stream, bool grid_all_tiles) {
     const uint32_t resident = pp_residency<B, V, I>();
     if (resident == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = (grid_all_tiles || p.num_tiles < resident) ? p.num_tiles : resident;
+    const uint32_t grid = grid_all_tiles ? p.num_tiles : persistent_grid(p.num_tiles, resident);
     hipLaunchKernelGGL((csdl_pp_u32<B, V, I>), dim3(grid), dim3(B), 0, stream, p);
     return hipGetLastError();
 }

@@ -367,7 +367,7 @@ template <int B, int V, bool I>
 static hipError_t launch_seg_pp(const SegParams& p, hipStream_t stream) {
     const uint32_t resident = seg_pp_residency<B, V, I>();
     if (resident == 0) return hipErrorInvalidDevice;
-    const uint32_t grid = p.num_tiles < resident ? p.num_tiles : resident;
+    const uint32_t grid = persistent_grid(p.num_tiles, resident);
     hipLaunchKernelGGL((segscan_pp_u32<B, V, I>), dim3(grid), dim3(B), 0, stream, p);
     return hipGetLastError();
 }
