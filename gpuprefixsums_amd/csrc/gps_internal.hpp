@@ -65,22 +65,12 @@ uint32_t resident_workgroups() {
     return r.workgroups;
 }
 
-#ifndef GPS_PP_BALANCED
-#define GPS_PP_BALANCED 0
-#endif
-
 // Grid of a persistent launch over `tiles` tiles with `resident` workgroups
-// resident. With balancing, the fewest workgroups that still need the same
-// number of rounds (ceil(tiles / resident)), so the last round is nearly full.
-inline uint32_t persistent_grid(uint32_t tiles, uint32_t resident) {
-    if (tiles <= resident) return tiles;
-#if GPS_PP_BALANCED
-    const uint32_t rounds = (tiles + resident - 1) / resident;
-    return (tiles + rounds - 1) / rounds;
-#else
-    return resident;
-#endif
-}
+// resident: every resident workgroup, or one per tile when there are fewer
+// tiles. (Trimming the grid so the last round of tiles is nearly full, the
+// fewest workgroups that need ceil(tiles / resident) rounds, measured within
+// 1.5% at 50.9 M elements and slower for the segmented kernels.)
+inline uint32_t persistent_grid(uint32_t tiles, uint32_t resident) { return tiles < resident ? tiles : resident; }
 
 // Compute units of the current device (queried once per process; 0 if the
 // query fails).
