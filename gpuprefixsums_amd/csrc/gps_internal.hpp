@@ -153,8 +153,7 @@ bool fp_shape_supported(TileShape s);
 // Carries of the FP scans (values match GPS_FP_* in gps.h).
 enum FpCarry : int { kFpPlain = 0, kFpPair = 1 };
 // persistent: resident workgroups loop over ticketed tiles, loading tile i+1
-// while tile i's lookback resolves (otherwise, and always for FP64 carries,
-// one workgroup per tile).
+// while tile i's lookback resolves (otherwise one workgroup per tile).
 hipError_t launch_scan_fp(TileShape s, bool bf16, int carry, bool persistent, const void* in, float* out,
                           uint64_t n, uint32_t vec_ok, uint64_t* status, uint64_t* ticket, uint32_t* err,
                           hipStream_t stream);
