@@ -233,6 +233,8 @@ int gps_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t max_batch, gps_mlp
     *out = nullptr;
     if (!in_dim || !hidden || !max_batch || in_dim > 4096 || hidden > 4096)
         return fail(GPS_ERR_INVALID, "bad MLP dimensions");
+    if (chunks_of(max_batch) > 65535u)  // the chunk index is the weight-gradient grid's y dimension
+        return fail(GPS_ERR_TOO_LARGE, "max_batch %u exceeds %u samples", max_batch, 65535u * kChunk);
     gps_mlp* m = new (std::nothrow) gps_mlp();
     if (!m) return fail(GPS_ERR_INVALID, "out of host memory");
     m->D = in_dim;

@@ -222,7 +222,10 @@ int gps_inclusive_scan_bf16_f32(gps_handle* h, const uint16_t* d_in, float* d_ou
  * log2 scan time from workload and configuration features, trained with MSE
  * and Adam by HIP kernels. Parameters are one flat FP32 array:
  * [W1 (hidden x in_dim), b1, W2 (hidden x hidden), b2, W3 (hidden), b3].
- * Inputs, targets and outputs are host arrays (the data set is small). */
+ * Inputs, targets and outputs are host arrays (the data set is small).
+ * in_dim and hidden at most 4096; max_batch at most 65535 * 128 samples
+ * (weight gradients sum the batch in chunks of 128 samples, added in chunk
+ * order, so every run sums in the same order). */
 typedef struct gps_mlp gps_mlp;
 uint64_t gps_mlp_param_count(uint32_t in_dim, uint32_t hidden);
 int gps_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t max_batch, gps_mlp** out);
