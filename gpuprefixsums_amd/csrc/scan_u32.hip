@@ -594,12 +594,23 @@ __global__ __launch_bounds__(BLOCK) void csdldf_u32(ScanParams p) {
 // i+1's loads) with csdldf_u32's lookback: bounded spins, a workgroup-wide
 // redundant reduction of the blocking tile and CAS insertion.
 //
-// Emulated deadlock: a stalled tile's workgroup holds back its publication
-// (and so its next ticket) until a successor has inserted its aggregate. The
-// successor tile is then held by another workgroup, which takes its ticket,
-// publishes and looks back without waiting on anyone, so it reaches the
-// stalled tile, gives up after spin_limit polls and inserts. stall_period >= 2
-// (host-checked) keeps that successor itself unstalled.
+// Emulated deadlock: a stalled tile's workgroup holds back the tile's
+// publication until a successor has inserted its aggregate. At stall period 2
+// it also holds back its next ticket, so the successor tile is held by
+// another workgroup, which takes its ticket, publishes and looks back without
+// waiting on anyone, reaches the stalled tile, gives up after spin_limit
+// polls and inserts. From period 3 the stalled workgroup first takes its next
+// ticket, loads that tile and (unless it stalls too) scans it and publishes
+// its aggregate, then waits, so the tile it holds never holds up a lookback.
+// Progress from period 3: take the lowest stalled tile s. A workgroup's next
+// tile is above its current one, and every other stalled tile is at least
+// s + 3, so tiles s + 1 and s + 2 (neither stalls) can only be held by s's
+// own workgroup (as its next tile, one of them) or by workgroups that are not
+// stalled, or be unticketed yet. A workgroup that is not stalled and holds
+// either looks back through s and inserts it; an unticketed one is taken by
+// a workgroup that is not stalled; the one case left is s + 1 as the stalled
+// workgroup's next tile and the last tile, where the stall is skipped.
+// stall_period >= 2 (host-checked) keeps tile + 1 unstalled.
 //
 // In-place safety: as in csdldf_u32, every publication is a returning
 // exchange whose result is checked (so waited on) before the barrier that
@@ -613,6 +624,36 @@ __device__ __forceinline__ void publish_incl_checked(const ScanParams& p, uint32
                                                      uint32_t epoch) {
     const uint64_t old = atomic_xchg_agent(p.status + kSlotWords * tile, make_status(value, kFlagIncl, epoch));
     if (status_flag(old, epoch) != (uint32_t)kFlagAgg) raise_device_error(p.err, kErrLookbackWatchdog);
+}
+
+// Emulated deadlock: tiles tile % stall_period == 0 that have a successor
+// (the split schedule's launches defer those tiles instead).
+__device__ __forceinline__ bool is_stalled_tile(const ScanParams& p, uint32_t tile) {
+    return p.stall_period && !p.split_pass && tile % p.stall_period == 0 && tile + 1u < p.num_tiles;
+}
+
+// One thread: waits until a successor has inserted this tile's word, bounded
+// by the device watchdog.
+__device__ __forceinline__ void stall_wait(const ScanParams& p, uint32_t tile, uint32_t epoch,
+                                           unsigned long long* s_count) {
+    uint32_t polls = 0;
+    while (status_flag(atomic_load_agent(p.status + kSlotWords * tile), epoch) == 0u) {
+        if (watchdog_expired(polls, p.err)) {
+            ++s_count[3];
+            raise_device_error(p.err, kErrStallWatchdog);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// One thread: the owner's aggregate (tile 0: inclusive) publication, a
+// returning exchange. Only the owner writes an inclusive word for a tile > 0.
+__device__ __forceinline__ void publish_own_word(const ScanParams& p, uint32_t tile, uint32_t tile_total,
+                                                 uint32_t epoch) {
+    const uint64_t old =
+        atomic_xchg_agent(p.status + kSlotWords * tile, make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
+    if (tile != 0 && status_flag(old, epoch) == (uint32_t)kFlagIncl) raise_device_error(p.err, kErrLookbackWatchdog);
 }
 
 template <int BLOCK, int VEC, bool INCL>
@@ -656,34 +697,74 @@ csdldf_pp_u32(ScanParams p) {
             load_guarded<VEC>(a, p.in, base, p.n);
     }
 
+    // Set when the current tile's aggregate was published while the previous
+    // tile stalled (stall-time prepublication, below).
+    bool prepublished = false;
     while (true) {
         const uint32_t run = wave_scan_run<VEC, INCL>(a);
         if (lane == 0) s_wave[wave] = run;
         __syncthreads();
         uint32_t wave_prefix, tile_total;
         combine_waves<C::kWaves>(s_wave, wave, wave_prefix, tile_total);
-        if (threadIdx.x == 0) {
-            if (p.stall_period && !p.split_pass && tile % p.stall_period == 0 && tile + 1u < p.num_tiles) {
-                uint32_t polls = 0;
-                while (status_flag(atomic_load_agent(p.status + kSlotWords * tile), epoch) == 0u) {
-                    if (watchdog_expired(polls, p.err)) {
-                        ++s_count[3];
-                        raise_device_error(p.err, kErrStallWatchdog);
-                        break;
+        const bool stalls = is_stalled_tile(p, tile);
+        uint32_t next;
+        bool has_next;
+        bool prefetched = false;
+        if (stalls && p.stall_period >= 3u) {
+            // Stall-time prepublication. The stalled workgroup takes its next
+            // ticket before it waits, loads that tile and, unless it stalls
+            // too, reduces it and publishes its aggregate, so no waiter is
+            // held up by the tile it holds. The stall is skipped when the next
+            // tile is this one's successor and the last tile: no other
+            // workgroup would be left to release it. (At period 2 the
+            // stalled workgroups could hold every successor between them, so
+            // period 2 keeps the plain order.)
+            if (threadIdx.x == 0) s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+            __syncthreads();
+            next = s_ticket[2];
+            has_next = next < p.num_tiles;
+            if (has_next) {
+                const uint64_t next_base = thread_base<BLOCK, VEC>(next);
+                if (p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n))
+                    load_vec<VEC>(b, p.in + next_base);
+                else
+                    load_guarded<VEC>(b, p.in, next_base, p.n);
+                prefetched = true;
+            }
+            const bool pre_next = has_next && !is_stalled_tile(p, next);
+            if (pre_next) {
+                // The next tile's total (its scan, next iteration, gives the
+                // same sum mod 2^32, and skips this publication).
+                const uint32_t ws = wave_sum(thread_sum<VEC>(b));
+                if (lane == 0) s_red[wave] = ws;
+                __syncthreads();
+                if (wave == 0) {
+                    const uint32_t total = wave_sum(lane < (uint32_t)C::kWaves ? s_red[lane] : 0u);
+                    if (lane == 0) {
+                        const uint64_t old =
+                            atomic_xchg_agent(p.status + kSlotWords * next, make_status(total, kFlagAgg, epoch));
+                        if (status_flag(old, epoch) == (uint32_t)kFlagIncl)
+                            raise_device_error(p.err, kErrLookbackWatchdog);
                     }
-                    __builtin_amdgcn_s_sleep(2);
                 }
             }
-            // Only this workgroup writes an inclusive word for a tile > 0.
-            const uint64_t old = atomic_xchg_agent(p.status + kSlotWords * tile,
-                                                   make_status(tile_total, tile ? kFlagAgg : kFlagIncl, epoch));
-            if (tile != 0 && status_flag(old, epoch) == (uint32_t)kFlagIncl)
-                raise_device_error(p.err, kErrLookbackWatchdog);
-            s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+            if (threadIdx.x == 0) {
+                if (!(next == tile + 1u && next + 1u == p.num_tiles)) stall_wait(p, tile, epoch, s_count);
+                publish_own_word(p, tile, tile_total, epoch);
+            }
+            __syncthreads();
+            prepublished = pre_next;
+        } else {
+            if (threadIdx.x == 0) {
+                if (stalls) stall_wait(p, tile, epoch, s_count);
+                if (!prepublished) publish_own_word(p, tile, tile_total, epoch);
+                s_ticket[2] = take_ticket(p.ticket, total_tickets).tile;
+            }
+            __syncthreads();
+            next = s_ticket[2];
+            has_next = next < p.num_tiles;
+            prepublished = false;
         }
-        __syncthreads();
-        const uint32_t next = s_ticket[2];
-        const bool has_next = next < p.num_tiles;
 
         // Lookback state (wave 0; uniform there). The first window is issued
         // ahead of the prefetch so the in-order vmcnt does not make its use
@@ -696,7 +777,7 @@ csdldf_pp_u32(ScanParams p) {
             const uint64_t v = atomic_load_agent(p.status + kSlotWords * (uint64_t)(idx >= 0 ? idx : 0));
             w = idx >= 0 ? v : make_status(0u, kFlagIncl, epoch);
         }
-        if (has_next) {
+        if (has_next && !prefetched) {
             const uint64_t next_base = thread_base<BLOCK, VEC>(next);
             if (p.vec_ok && tile_is_full<BLOCK, VEC>(next, p.n))
                 load_vec<VEC>(b, p.in + next_base);

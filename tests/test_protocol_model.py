@@ -176,6 +176,58 @@ def workgroup_program(mem, tickets, agg, epoch, num_tiles, period, spin_limit, s
         t = nxt
 
 
+def is_stall_tile(t, num_tiles, period):
+    return bool(period) and t % period == 0 and t + 1 < num_tiles
+
+
+def workgroup_program_prepublish(mem, tickets, agg, epoch, num_tiles, period, spin_limit, stats, result):
+    """Generator: the persistent decoupled-fallback kernel with stall-time
+    prepublication. A workgroup whose tile t stalls first takes its next
+    ticket and, unless that tile stalls too, publishes the next tile's
+    aggregate before it waits, so nobody waits on the tile it holds; the
+    stall is skipped when the next tile is t + 1 and the last tile (no other
+    workgroup would be left to release it). Period 2 keeps the plain order:
+    there the stalled workgroups could hold every successor between them."""
+    t = tickets[0]
+    tickets[0] += 1
+    yield
+    prepublished = False
+    while t < num_tiles:
+        if period >= 3 and is_stall_tile(t, num_tiles, period):
+            assert not prepublished
+            nxt = tickets[0]
+            tickets[0] += 1
+            yield
+            pre_next = nxt < num_tiles and not is_stall_tile(nxt, num_tiles, period)
+            if pre_next:
+                mem.status[nxt] = make_status(agg[nxt], AGG, epoch)  # exchange; nxt > t >= 0
+                stats["prepublished"] += 1
+                yield
+            if not (nxt == t + 1 and nxt + 1 == num_tiles):
+                while True:
+                    w = mem.status[t]
+                    yield
+                    if flag_of(w, epoch):
+                        stats["stall_released"] += 1
+                        break
+            else:
+                stats["stall_skipped"] += 1
+            mem.status[t] = make_status(agg[t], AGG if t else INCL, epoch)  # exchange
+            yield
+            yield from resolve(mem, t, agg, epoch, spin_limit, stats, result)
+            prepublished = pre_next
+            t = nxt
+            continue
+        if not prepublished:
+            yield from stall_then_publish(mem, t, agg, epoch, num_tiles, period, stats)
+        nxt = tickets[0]
+        tickets[0] += 1
+        yield
+        yield from resolve(mem, t, agg, epoch, spin_limit, stats, result)
+        prepublished = False
+        t = nxt
+
+
 def run_launch(mem, agg, epoch, period, spin_limit, resident, rng, stats):
     num_tiles = len(agg)
     result = [None] * num_tiles
@@ -196,13 +248,14 @@ def run_launch(mem, agg, epoch, period, spin_limit, resident, rng, stats):
     return result
 
 
-def run_persistent(mem, agg, epoch, period, spin_limit, grid, rng, stats):
+def run_persistent(mem, agg, epoch, period, spin_limit, grid, rng, stats, program=None):
     """All `grid` workgroups are resident from the start (the persistent
     grid is sized to what is resident) and share one ticket counter."""
     num_tiles = len(agg)
     result = [None] * num_tiles
     tickets = [0]
-    progs = {g: workgroup_program(mem, tickets, agg, epoch, num_tiles, period, spin_limit, stats, result)
+    program = program or workgroup_program
+    progs = {g: program(mem, tickets, agg, epoch, num_tiles, period, spin_limit, stats, result)
              for g in range(min(grid, num_tiles))}
     active = list(progs)
     steps = 0
@@ -391,6 +444,48 @@ def test_persistent_schedule_under_random_interleavings(seed):
             assert res[t] == acc, (seed, launch, t)
             acc = (acc + agg[t]) & MASK32
         assert all(flag_of(mem.status[t], epoch) == INCL for t in range(num_tiles))
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_prepublishing_persistent_schedule_under_random_interleavings(seed):
+    # Stall-time prepublication: the launch must finish with every prefix
+    # exact, at every period (2 keeps the plain order), grid and tile count,
+    # including the tile just before the last one stalling.
+    rng = random.Random(900 + seed)
+    num_tiles = rng.choice([1, 2, 3, 4, 9, 40, 130])
+    mem = Memory(200)
+    stats = Counter()
+    for launch in range(4):
+        epoch = launch + 1
+        agg = [rng.getrandbits(32) for _ in range(num_tiles)]
+        res = run_persistent(mem, agg, epoch, period=rng.choice([0, 2, 3, 4, 7, 8]), spin_limit=rng.choice([1, 2, 4]),
+                             grid=rng.choice([2, 3, 5, 17]), rng=rng, stats=stats,
+                             program=workgroup_program_prepublish)
+        acc = 0
+        for t in range(num_tiles):
+            assert res[t] == acc, (seed, launch, t)
+            acc = (acc + agg[t]) & MASK32
+        assert all(flag_of(mem.status[t], epoch) == INCL for t in range(num_tiles))
+
+
+def test_prepublishing_takes_its_branches_and_cuts_insertions():
+    """Across many launches: tiles are prepublished, the last-but-one stall
+    is skipped when its workgroup holds the last tile, and fewer tiles that
+    never stalled are reduced by a waiter than in the plain order."""
+    plain, pre = Counter(), Counter()
+    for seed in range(40):
+        for program, stats in ((workgroup_program, plain), (workgroup_program_prepublish, pre)):
+            rng = random.Random(3000 + seed)
+            num_tiles = rng.choice([5, 9, 24, 48])  # 5 and 9: the tile before the last stalls at period 3 or 7
+            agg = [rng.getrandbits(32) for _ in range(num_tiles)]
+            res = run_persistent(Memory(64), agg, 1, period=rng.choice([3, 7]), spin_limit=1,
+                                 grid=rng.choice([2, 3, 9]), rng=rng, stats=stats, program=program)
+            acc = 0
+            for t, r in enumerate(res):
+                assert r == acc
+                acc = (acc + agg[t]) & MASK32
+    assert pre["prepublished"] > 0 and pre["stall_skipped"] > 0, dict(pre)
+    assert pre["cas_win"] + pre["cas_lost_agg"] <= plain["cas_win"] + plain["cas_lost_agg"], (dict(pre), dict(plain))
 
 
 def test_persistent_schedule_exercises_the_fallback_branches():
