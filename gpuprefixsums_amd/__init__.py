@@ -282,6 +282,14 @@ class Graph:
         self.close()
 
 
+def status_memory() -> tuple:
+    """(held, free): bytes of uncached status memory the library holds on all
+    devices, and how many of them sit on its free lists for reuse."""
+    held, free = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    check(load().gps_status_memory(ctypes.byref(held), ctypes.byref(free)))
+    return held.value, free.value
+
+
 # ---- utility kernels -----------------------------------------------------
 
 def fill_u32(d_ptr, n: int, value: int, stream=None) -> None:

@@ -60,6 +60,7 @@ SIGNATURES = {
     "gps_last_error": (ctypes.c_char_p, []),
     "gps_create": (_int, [_int, _u64, ctypes.POINTER(GpsTuning), ctypes.POINTER(_vp)]),
     "gps_destroy": (_int, [_vp]),
+    "gps_status_memory": (_int, [ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     "gps_get_tuning": (_int, [_vp, ctypes.POINTER(GpsTuning), ctypes.POINTER(_u32)]),
     "gps_inclusive_scan_u32": (_int, [_vp, _vp, _vp, _u64, _vp]),
     "gps_exclusive_scan_u32": (_int, [_vp, _vp, _vp, _u64, _vp]),

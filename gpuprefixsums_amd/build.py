@@ -41,6 +41,7 @@ LIB_SOURCES = [
     "survey.hip",
     "capi.cpp",
     "capi_ext.cpp",
+    "status_memory.cpp",
     "dispatchers.cpp",
     "noalloc.hip",
 ]

@@ -84,8 +84,7 @@ void free_handle(gps_handle* h) {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(h->device);
-    (void)hipFree(h->status);
-    (void)hipFree(h->ticket);
+    gps::status_block_release(h->device, h->status_block);
     (void)hipFree(h->err);
     (void)hipFree(h->counters);
     (void)hipFree(h->tile_sums);
@@ -227,8 +226,13 @@ int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** 
     h->size_adaptive = size_adaptive;
     h->max_tiles = (uint32_t)tiles;
     hipError_t e = hipGetDevice(&h->device);
-    if (e == hipSuccess) e = hipMalloc(&h->status, tiles * gps::kSlotWords * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc(&h->ticket, sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = gps::status_block_acquire(h->device, gps::kTicketBytes + tiles * gps::kSlotWords * sizeof(uint64_t),
+                                      h->status_block);
+    if (e == hipSuccess) {
+        h->ticket = static_cast<uint64_t*>(h->status_block.ptr);
+        h->status = reinterpret_cast<uint64_t*>(static_cast<char*>(h->status_block.ptr) + gps::kTicketBytes);
+    }
     if (e == hipSuccess) e = hipMalloc(&h->err, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&h->counters, gps::kCounterWords * sizeof(unsigned long long));
     if (e == hipSuccess && algo == GPS_ALGO_RTS) e = hipMalloc(&h->tile_sums, tiles * sizeof(uint32_t));

@@ -13,6 +13,29 @@
 // it so every entry point of a handle, whatever shape it runs, has its words.
 constexpr uint32_t kMinTileElems = 1024;
 
+namespace gps {
+
+// A handle's ticket word (first 256 bytes) and status slots (the rest), in
+// one block of device memory; see status_memory.cpp.
+struct StatusBlock {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+    bool uncached = false;
+};
+constexpr uint64_t kTicketBytes = 256;
+
+// Takes a block of at least `bytes` for `device` (the current device):
+// uncached memory from the per-device free list or newly allocated, or
+// ordinary memory when GPS_STATUS_MEMORY=plain.
+hipError_t status_block_acquire(int device, uint64_t bytes, StatusBlock& out);
+// Returns an uncached block to the free list (never to the runtime); frees an
+// ordinary one.
+void status_block_release(int device, const StatusBlock& b);
+// Bytes of uncached blocks allocated so far, and of those on the free lists.
+void status_memory_stats(uint64_t* held, uint64_t* free_bytes);
+
+}  // namespace gps
+
 struct gps_handle {
     int algo = 0;
     uint64_t max_n = 0;
@@ -23,6 +46,7 @@ struct gps_handle {
     bool size_adaptive = false;  // CSDL with default shape and schedule: both picked per call from n
     uint32_t max_tiles = 0;     // status slots allocated (one per kMinTileElems elements)
     int device = 0;
+    gps::StatusBlock status_block;  // holds ticket and status
     uint64_t* status = nullptr;
     uint64_t* ticket = nullptr;
     uint32_t* err = nullptr;

@@ -88,10 +88,18 @@ const char* gps_last_error(void);
 
 /* Creates a handle that owns the scratch for scans of up to max_n elements on
  * the current HIP device (ChainedScanDecoupledLookbackDispatcher.cuh:28-34).
- * tuning may be NULL. */
+ * tuning may be NULL. The ticket and status words (max_n / 8 + 256 bytes)
+ * are in uncached device memory, which the library never returns to the
+ * runtime: gps_destroy puts the block on a per-device free list that later
+ * handles reuse. Setting GPS_STATUS_MEMORY=plain in the environment before
+ * gps_create gives the handle ordinary device memory instead (freed by
+ * gps_destroy; the scans run about 2% slower at 2^28 elements). */
 int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** out);
 /* (...Dispatcher.cuh:158-163) */
 int gps_destroy(gps_handle* h);
+/* Bytes of uncached status memory the library holds on all devices, and how
+ * many of them are on the free lists (no reference counterpart). */
+int gps_status_memory(uint64_t* held_bytes, uint64_t* free_bytes);
 /* Effective tuning and tile size (elements per workgroup) of a handle. A
  * GPS_ALGO_CSDL handle created without a tile shape or schedule picks both
  * per call from n (one workgroup per tile, and smaller tiles, below 2^24
