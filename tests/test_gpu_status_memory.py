@@ -108,3 +108,30 @@ def test_plain_memory_is_not_held():
     with gps.ScanHandle(gps.ALGO_CSDL, 1 << 20):
         assert gps.status_memory() == (held0, free0)
     assert gps.status_memory() == (held0, free0)
+
+
+@pytest.mark.parametrize("kind", ["uncached", "plain"])
+def test_fp_pair_scan_in_either_memory(kind):
+    # the FP scans publish both status words of a tile in one 16-byte access
+    use_memory(kind)
+    n = (1 << 22) + 12345
+    x = oracle.gen_f32_normal(4, n)
+    y64 = oracle.scan_f32_f64(x)
+    S = oracle.abs_scan_f32_f64(x)
+    d_in = torch.from_numpy(x).cuda()
+    with gps.ScanHandle(gps.ALGO_CSDL, n) as h:
+        for comp in (gps.FP_PAIR, gps.FP_PLAIN):
+            out = torch.full((n + 8,), -7.0, dtype=torch.float32, device="cuda")
+            for _ in range(2):
+                h.inclusive_f32(d_in, out, n, comp)
+            sync()
+            assert h.device_error() == 0
+            y = out.cpu().numpy()
+            assert np.all(y[n:] == -7.0), "wrote past n"
+            err = np.abs(y[:n].astype(np.float64) - y64)
+            if comp == gps.FP_PAIR:
+                bound = 2.0 ** -23 * np.abs(y64) + 2.0 ** -36 * S
+            else:
+                # tile index at the smallest FP tile (4096): looser than the kernel's own
+                bound = (64 + 2 * (np.arange(n) // 4096 + 1)) * 2.0 ** -24 * S
+            assert np.all(err <= bound), f"{kind} comp={comp}: {int(np.sum(err > bound))} elements over the bound"
