@@ -81,13 +81,20 @@ __device__ __forceinline__ float readlane_f(float x, int lane) {
 // agreeing words belong to the same publication, and a read that tears between
 // two publications shows two flags and counts as not ready.
 //
-// Both words move in one 16-byte access (global_load/store_dwordx4): volatile,
-// so the compiler neither caches nor merges them, and system-scope coherent
-// (sc0 sc1), which bypasses the XCD L2s as the agent-scope atomics do. One
-// request per tile and lane instead of two: at 2^27 the pair scan measured
-// 5393-5466 -> 5741-5751 GB/s (FP32) and the plain one 5754-5803 -> 6138-6158.
+// Both words move in one 16-byte access, one request per tile and lane
+// instead of two. The store is a buffer_store_dwordx4 with the cache policy
+// of the agent-scope atomic stores (sc1: written through past the XCD L2).
+// The load is a volatile global_load_dwordx4, so the compiler re-issues it on
+// every poll, and system-scope coherent (sc0 sc1). At 2^27 against two 8-byte
+// atomics per access: FP32 pairs 5393-5466 -> 5895-5950 GB/s, BF16 pairs
+// 5148-5156 -> 5310-5417, FP32 plain 5754-5803 -> 6090-6136. (A volatile
+// store, sc0 sc1, gave 5672-5845 / 5126-5131; an sc1 buffer load instead of
+// the volatile one measured the same for FP32 and 3% slower for BF16.)
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_status __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) volatile u64x2 status_pair_t;
+constexpr int kCachePolicySc1 = 16;        // buffer instruction cache-policy bits: sc1
+constexpr int kBufferRsrcWord3 = 0x00020000;  // raw 32-bit buffer resource, no swizzle
 
 __device__ __forceinline__ void load_pair_words(const uint64_t* slot, uint64_t& w0, uint64_t& w1) {
     const u64x2 v = *(const status_pair_t*)(slot);
@@ -97,9 +104,12 @@ __device__ __forceinline__ void load_pair_words(const uint64_t* slot, uint64_t& 
 
 template <bool COMP>
 __device__ __forceinline__ void publish_ff(uint64_t* status, uint32_t tile, ff v, uint64_t flag, uint32_t epoch) {
-    const u64x2 w = {make_status(__builtin_bit_cast(uint32_t, v.s), flag, epoch),
-                     make_status(COMP ? __builtin_bit_cast(uint32_t, v.c) : 0u, flag, epoch)};
-    *(status_pair_t*)(status + kSlotWords * tile) = w;
+    const uint64_t w0 = make_status(__builtin_bit_cast(uint32_t, v.s), flag, epoch);
+    const uint64_t w1 = make_status(COMP ? __builtin_bit_cast(uint32_t, v.c) : 0u, flag, epoch);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(status + kSlotWords * tile, 0, 16, kBufferRsrcWord3);
+    const u32x4_status d = {(uint32_t)w0, (uint32_t)(w0 >> 32), (uint32_t)w1, (uint32_t)(w1 >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(d, rs, 0, 0, kCachePolicySc1);
 }
 
 // Window lookback over pair words (one full wave); returns the exclusive
