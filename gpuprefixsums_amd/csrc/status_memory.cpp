@@ -69,8 +69,15 @@ hipError_t status_block_acquire(int device, uint64_t bytes, StatusBlock& out) {
         }
     }
     void* p = nullptr;
-    const hipError_t e = hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached);
-    if (e != hipSuccess) return e;
+    if (hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached) != hipSuccess) {
+        // A device or driver without uncached allocations: ordinary memory,
+        // freed normally (the scans are correct in either).
+        (void)hipGetLastError();
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return e;
+        out = StatusBlock{p, bytes, false};
+        return hipSuccess;
+    }
     {
         std::lock_guard<std::mutex> lock(P.mu);
         P.held += bytes;
