@@ -102,6 +102,8 @@ __device__ __forceinline__ void load_pair_words(const uint64_t* slot, uint64_t& 
     w1 = v.y;
 }
 
+// Called by one lane at a time (the buffer resource is scalar: it takes the
+// address of the first active lane).
 template <bool COMP>
 __device__ __forceinline__ void publish_ff(uint64_t* status, uint32_t tile, ff v, uint64_t flag, uint32_t epoch) {
     const uint64_t w0 = make_status(__builtin_bit_cast(uint32_t, v.s), flag, epoch);
