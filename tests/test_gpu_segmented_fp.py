@@ -63,6 +63,14 @@ def offsets_cases(tile):
     }
     # a long run of segments that all start inside one tile, then one spanning many tiles
     cases["dense_then_long"] = np.r_[np.arange(0, 4000, 4), [4000 + 9 * tile + 17]].astype(np.uint64)
+    # heads at the edges of 32-element runs (and of 16-element ones), two heads in one run, and runs
+    # without a head between them: the lane-blocked kernel's first pass sums a run from its last
+    # head, and its output pass resets only in lanes whose run holds a head
+    pos = []
+    for r in range(3 * 64):
+        b = 32 * r
+        pos += [[b], [b + 1], [b + 31], [b + 30], [b + 3, b + 29], [b + 15, b + 16], []][r % 7]
+    cases["lane_run_edges"] = np.r_[sorted(set([0] + pos)), [32 * 3 * 64 + 2 * tile + 7]].astype(np.uint64)
     return cases
 
 
