@@ -17,10 +17,12 @@ from ._lib import PKG, GpsError
 CEILING_LIB_PATH = PKG / "libgps_ceiling.so"
 
 (COPY_64X1, COPY_256X4, COPY_1024X8, TICKET_1024X4, TICKET_1024X8, COPY_LDS_256X4, READ_1024X1, WRITE_256X1,
- MEMCPY, MEMSET, TICKET_1024X4_STATUS, TICKET_1024X8_STATUS) = range(12)
-KINDS = 12
+ MEMCPY, MEMSET, TICKET_1024X4_STATUS, TICKET_1024X8_STATUS, TICKET_1024X4_STATUS_UC,
+ TICKET_1024X8_STATUS_UC) = range(14)
+KINDS = 14
 COPY_KINDS = (COPY_64X1, COPY_256X4, COPY_1024X8, TICKET_1024X4, TICKET_1024X8, COPY_LDS_256X4, MEMCPY,
-              TICKET_1024X4_STATUS, TICKET_1024X8_STATUS)
+              TICKET_1024X4_STATUS, TICKET_1024X8_STATUS, TICKET_1024X4_STATUS_UC, TICKET_1024X8_STATUS_UC)
+# kinds that use the caller's scratch (ticket word, and status slots for the status kinds)
 TICKETED = (TICKET_1024X4, TICKET_1024X8, TICKET_1024X4_STATUS, TICKET_1024X8_STATUS)
 
 _vp, _u64, _int = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int

@@ -12,7 +12,11 @@
 // contiguous run of 64 * VEC vectors at w * 64 * VEC, and its vector k is the
 // 1 KiB at k * 64 inside that run (one dwordx4 per lane).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
+
+#include <map>
+#include <mutex>
 
 #include "../../include/gps_ceiling.h"
 #include "handle.hpp"
@@ -88,7 +92,9 @@ __global__ __launch_bounds__(BLOCK) void copy_ticketed(const u32x4* in, u32x4* o
 // poll; its ballot keeps the loads live) and publishes the inclusive word
 // before the stores. Status words are 8 bytes, one per 128-byte slot, written
 // and read with relaxed device-scope atomics, as in the scan.
-template <int BLOCK, int VEC>
+// UNCACHED only tells the kernel traces apart: the words live wherever the
+// caller's pointers point (uncached device memory for the _UC kinds).
+template <int BLOCK, int VEC, bool UNCACHED>
 __global__ __launch_bounds__(BLOCK) void copy_ticketed_status(const u32x4* in, u32x4* out, uint32_t tiles,
                                                               uint32_t* ticket, uint64_t* status) {
     constexpr uint64_t kSlot = 16;  // words per 128-byte slot
@@ -202,24 +208,61 @@ constexpr Kind kKinds[GPS_CEIL_KINDS] = {
     {"hipMemsetD32Async", 1, 1},
     {"ticketed copy 1024x4 + status", 1024, 4},
     {"ticketed copy 1024x8 + status", 1024, 8},
+    {"ticketed copy 1024x4 + uncached status", 1024, 4},
+    {"ticketed copy 1024x8 + uncached status", 1024, 8},
 };
 
 uint64_t granule_bytes(int kind) { return (uint64_t)kKinds[kind].block * kKinds[kind].vec * 16u; }
 
-bool ticketed(int kind) {
-    return kind == GPS_CEIL_TICKET_1024X4 || kind == GPS_CEIL_TICKET_1024X8 || kind == GPS_CEIL_TICKET_1024X4_STATUS ||
-           kind == GPS_CEIL_TICKET_1024X8_STATUS;
+bool uncached_status(int kind) {
+    return kind == GPS_CEIL_TICKET_1024X4_STATUS_UC || kind == GPS_CEIL_TICKET_1024X8_STATUS_UC;
 }
 
-bool with_status(int kind) { return kind == GPS_CEIL_TICKET_1024X4_STATUS || kind == GPS_CEIL_TICKET_1024X8_STATUS; }
+bool ticketed(int kind) {
+    return kind == GPS_CEIL_TICKET_1024X4 || kind == GPS_CEIL_TICKET_1024X8 || kind == GPS_CEIL_TICKET_1024X4_STATUS ||
+           kind == GPS_CEIL_TICKET_1024X8_STATUS || uncached_status(kind);
+}
+
+bool with_status(int kind) {
+    return kind == GPS_CEIL_TICKET_1024X4_STATUS || kind == GPS_CEIL_TICKET_1024X8_STATUS || uncached_status(kind);
+}
 
 // Scratch layout: the ticket word alone in the first 256 bytes, then one
 // 128-byte status slot per tile for the kinds with status traffic.
 constexpr uint64_t kTicketBytes = 256;
 
-uint64_t scratch_bytes(int kind, uint64_t bytes) {
+uint64_t needed_bytes(int kind, uint64_t bytes) {
     if (!ticketed(kind)) return 0;
     return kTicketBytes + (with_status(kind) ? bytes / granule_bytes(kind) * 128u : 0u);
+}
+
+uint64_t scratch_bytes(int kind, uint64_t bytes) { return uncached_status(kind) ? 0 : needed_bytes(kind, bytes); }
+
+// The uncached-status kinds' scratch: one block of uncached device memory per
+// device, zeroed when allocated, grown by allocating a larger one. Uncached
+// memory is never freed once a kernel has read it (memory allocated after
+// such a free has held wrong values on this platform), so a replaced block
+// stays allocated.
+std::mutex g_uc_mu;
+std::map<int, std::pair<void*, uint64_t>> g_uc_blocks;
+
+hipError_t uncached_scratch(uint64_t bytes, void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_uc_mu);
+    auto& b = g_uc_blocks[dev];
+    if (b.second < bytes) {
+        void* p = nullptr;
+        e = hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached);
+        if (e != hipSuccess) return e;
+        e = hipMemset(p, 0, bytes);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) return e;
+        b = {p, bytes};
+    }
+    *out = b.first;
+    return hipSuccess;
 }
 
 }  // namespace
@@ -271,7 +314,10 @@ extern "C" int gps_ceiling_run(int kind, const void* src, void* dst, uint64_t by
         case GPS_CEIL_TICKET_1024X4:
         case GPS_CEIL_TICKET_1024X8:
         case GPS_CEIL_TICKET_1024X4_STATUS:
-        case GPS_CEIL_TICKET_1024X8_STATUS: {
+        case GPS_CEIL_TICKET_1024X8_STATUS:
+        case GPS_CEIL_TICKET_1024X4_STATUS_UC:
+        case GPS_CEIL_TICKET_1024X8_STATUS_UC: {
+            if (uncached_status(kind)) HIP_TRY(uncached_scratch(needed_bytes(kind, bytes), &scratch));
             if (!scratch) return fail(GPS_ERR_INVALID, "the ticketed copies need scratch (gps_ceiling_scratch_bytes)");
             if (reinterpret_cast<uintptr_t>(scratch) & 255u) return fail(GPS_ERR_INVALID, "scratch must be 256-byte aligned");
             const uint32_t cus = gps::device_cus();
@@ -285,9 +331,17 @@ extern "C" int gps_ceiling_run(int kind, const void* src, void* dst, uint64_t by
             else if (kind == GPS_CEIL_TICKET_1024X8)
                 hipLaunchKernelGGL((copy_ticketed<1024, 8>), pgrid, dim3(1024), 0, s, in, out, nt, ticket);
             else if (kind == GPS_CEIL_TICKET_1024X4_STATUS)
-                hipLaunchKernelGGL((copy_ticketed_status<1024, 4>), pgrid, dim3(1024), 0, s, in, out, nt, ticket, status);
+                hipLaunchKernelGGL((copy_ticketed_status<1024, 4, false>), pgrid, dim3(1024), 0, s, in, out, nt, ticket,
+                                   status);
+            else if (kind == GPS_CEIL_TICKET_1024X8_STATUS)
+                hipLaunchKernelGGL((copy_ticketed_status<1024, 8, false>), pgrid, dim3(1024), 0, s, in, out, nt, ticket,
+                                   status);
+            else if (kind == GPS_CEIL_TICKET_1024X4_STATUS_UC)
+                hipLaunchKernelGGL((copy_ticketed_status<1024, 4, true>), pgrid, dim3(1024), 0, s, in, out, nt, ticket,
+                                   status);
             else
-                hipLaunchKernelGGL((copy_ticketed_status<1024, 8>), pgrid, dim3(1024), 0, s, in, out, nt, ticket, status);
+                hipLaunchKernelGGL((copy_ticketed_status<1024, 8, true>), pgrid, dim3(1024), 0, s, in, out, nt, ticket,
+                                   status);
             break;
         }
         case GPS_CEIL_COPY_LDS_256X4:

@@ -37,7 +37,11 @@ enum gps_ceiling_kind {
                                            an aggregate and an inclusive word published and one 64-word
                                            lookback poll, all relaxed device-scope atomics, no waits */
     GPS_CEIL_TICKET_1024X8_STATUS = 11, /* the same at 1024x8 */
-    GPS_CEIL_KINDS = 12
+    GPS_CEIL_TICKET_1024X4_STATUS_UC = 12, /* kind 10 with its ticket and status words in uncached
+                                              device memory held by the library, as the scans keep
+                                              theirs (no caller scratch) */
+    GPS_CEIL_TICKET_1024X8_STATUS_UC = 13, /* the same at 1024x8 */
+    GPS_CEIL_KINDS = 14
 };
 
 /* Bytes per tile of `kind`: `bytes` passed to gps_ceiling_run must be a
@@ -48,17 +52,21 @@ int gps_ceiling_granule(int kind, uint64_t* granule);
 const char* gps_ceiling_name(int kind);
 
 /* Device scratch bytes `kind` needs for `bytes` of data: 0 for the one-shot
- * kinds; for the ticketed kinds a ticket word (256 bytes) plus, for the
- * status kinds, one 128-byte status slot per tile. */
+ * kinds and the uncached-status kinds; for the other ticketed kinds a ticket
+ * word (256 bytes) plus, for the status kinds, one 128-byte status slot per
+ * tile. */
 int gps_ceiling_scratch_bytes(int kind, uint64_t bytes, uint64_t* scratch);
 
 /* One launch of probe `kind` on `stream`: copies `bytes` from src to dst
  * (copy kinds and MEMCPY), reads `bytes` of src (READ; dst receives at most one
  * word, so it needs 4 bytes), or writes `bytes` to dst (WRITE and MEMSET; src
  * unused). src and dst are 16-byte aligned device buffers that do not
- * overlap. The ticketed kinds need `scratch`: 256-byte aligned, at least
- * gps_ceiling_scratch_bytes, zero before the first launch; each launch
- * leaves its ticket word zero again. */
+ * overlap. The ticketed kinds other than the uncached-status ones need
+ * `scratch`: 256-byte aligned, at least gps_ceiling_scratch_bytes, zero
+ * before the first launch; each launch leaves its ticket word zero again.
+ * The uncached-status kinds ignore `scratch` and use a block of uncached
+ * device memory per device that the library allocates on first use (or
+ * when a larger run needs more) and never frees. */
 int gps_ceiling_run(int kind, const void* src, void* dst, uint64_t bytes, void* scratch, void* stream);
 
 #ifdef __cplusplus
