@@ -84,6 +84,10 @@ void free_handle(gps_handle* h) {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(h->device);
+    // hipFree waits for the device; an uncached status block goes back to a
+    // free list instead, so wait here: no launch still in flight may touch
+    // the block once another handle can take it.
+    (void)hipDeviceSynchronize();
     gps::status_block_release(h->device, h->status_block);
     (void)hipFree(h->err);
     (void)hipFree(h->counters);
