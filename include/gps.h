@@ -95,7 +95,8 @@ const char* gps_last_error(void);
  * gps_create gives the handle ordinary device memory instead (freed by
  * gps_destroy; the scans run about 2% slower at 2^28 elements). */
 int gps_create(int algo, uint64_t max_n, const gps_tuning* tuning, gps_handle** out);
-/* (...Dispatcher.cuh:158-163) */
+/* Waits for the handle's device to finish its work, then releases the
+ * handle (...Dispatcher.cuh:158-163). */
 int gps_destroy(gps_handle* h);
 /* Bytes of uncached status memory the library holds on all devices, and how
  * many of them are on the free lists (no reference counterpart). */
